@@ -1,0 +1,285 @@
+/*
+ * qikkdb_amd.h - C ABI of the MI355X (gfx950) WHERE -> reconstruct -> GROUP BY path.
+ *
+ * Every entry point takes plain pointers, sizes and an optional HIP stream handle
+ * (`void* stream`, NULL = the device's default stream).  Device buffers are raw device
+ * pointers.  Buffers that an entry point allocates for the caller ("Keep" outputs,
+ * GROUP BY results) come from qk_malloc and are released by the caller with qk_free,
+ * which is the ownership rule of the reference's GPUMemory::alloc/free
+ * (qikkDB/QueryEngine/GPUCore/GPUMemory.cuh:89-110, GPUMemory.cpp:15-20).
+ *
+ * Each entry point returns a status code.  Codes 0..5 carry the meaning of the
+ * reference's QueryEngineErrorType (qikkDB/QueryEngine/GPUError.h:17-28); the C++ drop-in
+ * wrappers under include/qikkdb/ rethrow them as query_engine_error.  The text of the last
+ * failure on the calling thread is returned by qk_last_error_message().
+ */
+#ifndef QIKKDB_AMD_H
+#define QIKKDB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------ */
+enum qk_status {
+    QK_SUCCESS = 0,           /* GPU_EXTENSION_SUCCESS */
+    QK_ERROR = 1,             /* GPU_EXTENSION_ERROR: invalid argument or unsupported form */
+    QK_DIVISION_BY_ZERO = 2,  /* GPU_DIVISION_BY_ZERO_ERROR */
+    QK_INTEGER_OVERFLOW = 3,  /* GPU_INTEGER_OVERFLOW_ERROR */
+    QK_HASH_TABLE_FULL = 4,   /* GPU_HASH_TABLE_FULL */
+    QK_UNKNOWN_AGG_FUN = 5,   /* GPU_UNKNOWN_AGG_FUN */
+    QK_RUNTIME_ERROR = 64,    /* a HIP runtime call failed (the reference aborts, GPUError.cpp:37) */
+    QK_OUT_OF_MEMORY = 65     /* device allocation failed (the reference throws std::out_of_range) */
+};
+
+/* ---- column types: numbering of DataType (qikkDB/DataType.h:18-25) --------------------- */
+enum qk_type { QK_INT = 8, QK_LONG = 9, QK_FLOAT = 10, QK_DOUBLE = 11, QK_INT8 = 15 };
+
+/* ---- filter operators: the FilterConditions functors (GPUFilterConditions.cuh:9-110) ---- */
+enum qk_filter_op {
+    QK_GREATER = 0,
+    QK_GREATER_EQUAL = 1,
+    QK_LESS = 2,
+    QK_LESS_EQUAL = 3,
+    QK_EQUAL = 4,
+    QK_NOT_EQUAL = 5,
+    QK_LOGICAL_AND = 6,
+    QK_LOGICAL_OR = 7
+};
+
+/* ---- operand kinds: column pointer or scalar (MaybeDeref.cuh:3-14) --------------------- */
+enum qk_operand { QK_COLUMN = 0, QK_CONST = 1 };
+
+/* ---- aggregation functors (AggregationFunctions.cuh:17-366) ----------------------------
+ * QK_AGG_COUNT counts rows whose value is not NULL (all rows when no value null mask is
+ * given, which is how the reference evaluates COUNT(*), GpuSqlDispatcherAggregationFunctions.h
+ * :388-406).  QK_AGG_COUNT_ALL counts every row whatever the value null mask says, so that a
+ * table holding several aggregates can carry COUNT(*) next to a nullable value column.     */
+enum qk_agg {
+    QK_AGG_MIN = 0,
+    QK_AGG_MAX = 1,
+    QK_AGG_SUM = 2,
+    QK_AGG_AVG = 3,
+    QK_AGG_COUNT = 4,
+    QK_AGG_NONE = 5,
+    QK_AGG_COUNT_ALL = 6
+};
+
+/* ======================================================================================
+ * Runtime: device binding, memory, streams
+ * ====================================================================================== */
+
+/* Bind the calling thread to `device` and initialise the library on it.
+ * Replaces Context::getInstance() + bindDeviceToContext (Context.cu:20-100, :151-162). */
+int qk_init(int device);
+int qk_device_count(int* count);
+/* Text of the last failure on the calling thread ("" when none). */
+const char* qk_last_error_message(void);
+
+/* Pooled device allocation.  Replaces GPUMemory::alloc / GPUMemory::free (GPUMemory.cuh:89-110,
+ * GPUMemory.cpp:15-20) over the reference's CudaMemAllocator (CudaMemAllocator.cu:109-322). */
+int qk_malloc(void** ptr, size_t bytes);
+int qk_free(void* ptr);
+/* Bytes handed out and bytes held in the pool's free lists. */
+int qk_mem_stats(size_t* in_use, size_t* cached);
+/* Return every cached block to the driver. */
+int qk_mem_release_cached(void);
+
+/* Copies: GPUMemory::copyHostToDevice / copyDeviceToHost / copyDeviceToDevice
+ * (GPUMemory.cuh:113-189).  qk_memcpy_dtoh returns after the bytes are on the host. */
+int qk_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+int qk_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+int qk_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+int qk_memset(void* dst, int byte_value, size_t bytes, void* stream);
+
+int qk_stream_create(void** stream);
+int qk_stream_destroy(void* stream);
+int qk_stream_synchronize(void* stream);
+int qk_device_synchronize(void);
+
+/* ======================================================================================
+ * Filter (WHERE)
+ * ====================================================================================== */
+
+/* out_mask[i] = OP(lhs[i] or lhs, rhs[i] or rhs) && !null(i), one int8 0/1 per row.
+ * Comparison typing follows the C++ usual arithmetic conversions of (ltype, rtype).  A
+ * QK_CONST operand points to a HOST scalar of its type.  null_mask (LSB-first uint64 words,
+ * NULL = no nulls) is the OR of the operands' masks, as the dispatcher prepares it.
+ * const-const fills the mask.  Replaces GPUBinary<OP,int8_t,L,R>::Binary
+ * (GPUBinary.cuh:554-579) over kernel_filter (GPUFilter.cuh:22-44). */
+int qk_filter(int32_t op, int32_t ltype, int32_t lkind, const void* lhs, int32_t rtype,
+              int32_t rkind, const void* rhs, int32_t n, const uint64_t* null_mask,
+              int8_t* out_mask, void* stream);
+
+/* Same predicate, written as a bit mask: bit (i % 64) of word i / 64 is row i's result.
+ * No reference counterpart (the reference only has int8 masks); fed to the bit-mask forms
+ * of the GROUP BY below. */
+int qk_filter_bits(int32_t op, int32_t ltype, int32_t lkind, const void* lhs, int32_t rtype,
+                   int32_t rkind, const void* rhs, int32_t n, const uint64_t* null_mask,
+                   uint64_t* out_bits, void* stream);
+
+/* out[i] = !in[i] && !null(i).  Replaces kernel_operator_not (GPULogic.cuh:17-33). */
+int qk_logical_not(const int8_t* in, int32_t n, const uint64_t* null_mask, int8_t* out,
+                   void* stream);
+
+/* ======================================================================================
+ * Reconstruct (stream compaction)
+ * ====================================================================================== */
+
+/* Inclusive (inclusive != 0) or exclusive int32 prefix sum of an int8 mask.  Replaces
+ * GPUReconstruct::PrefixSum / PrefixSumExclusive (GPUReconstruct.cuh:572-601). */
+int qk_prefix_sum(const int8_t* mask, int32_t n, int32_t* out, int32_t inclusive, void* stream);
+
+/* Sum of the int8 mask values, written to *out_host.  Replaces GPUReconstruct::Sum
+ * (GPUReconstruct.cuh:604-624) on masks. */
+int qk_mask_sum(const int8_t* mask, int32_t n, int64_t* out_host, void* stream);
+
+/* Stable compaction of `in` (n elements of elem_size bytes, 1/2/4/8) by mask, with the
+ * null bits of in_null carried along.  Allocates *out (and *out_null when both out_null and
+ * in_null are given); *out_count is the number of kept rows.  mask == NULL aliases the input
+ * (*out = in, no ownership passes); zero kept rows or n <= 0 give *out = NULL.  Replaces
+ * GPUReconstruct::reconstructColKeep (GPUReconstruct.cuh:271-355). */
+int qk_reconstruct_col_keep(int32_t elem_size, const void* in, const int8_t* mask, int32_t n,
+                            const uint64_t* in_null, void** out, int32_t* out_count,
+                            uint64_t** out_null, void* stream);
+
+/* Indices (int32 when index_size == 4, int64 when 8) of the rows whose mask is non-zero, in
+ * row order; allocates *out.  Replaces GPUReconstruct::GenerateIndexesKeep
+ * (GPUReconstruct.cuh:517-566). */
+int qk_generate_indexes_keep(const int8_t* mask, int32_t n, int32_t index_size, void** out,
+                             int32_t* out_count, void* stream);
+
+/* One byte per row (bit 0 used) -> LSB-first uint64 bit mask of ceil(n/64) words; allocates
+ * *out.  Replaces GPUReconstruct::CompressNullMask (GPUReconstruct.cu:756-764). */
+int qk_compress_null_mask(const uint8_t* in, int32_t n, uint64_t** out, void* stream);
+
+/* WHERE + reconstruct in one pass (no reference counterpart: the reference runs the filter,
+ * then one scan + scatter per column, GpuSqlDispatcherAggregationFunctions.h:422-442 and
+ * :532-536).  Rows passing every predicate are copied, in row order, from each of the ncols
+ * input columns into caller-provided outputs of capacity n; *out_count receives the kept
+ * row count.  Predicates are described by qk_predicate below. */
+struct qk_predicate;
+int qk_where_compact(int32_t npreds, const struct qk_predicate* preds, int32_t ncols,
+                     const int32_t* elem_sizes, const void* const* cols,
+                     const uint64_t* const* col_nulls, int32_t n, void* const* outs,
+                     uint64_t* const* out_nulls, int32_t* out_count, void* stream);
+
+/* ======================================================================================
+ * GROUP BY
+ * ====================================================================================== */
+
+/* One comparison of a conjunctive WHERE, evaluated inside the GROUP BY kernel.
+ * column OP (rhs_const | rhs_column); a NULL operand makes the row fail (GPUFilter.cuh:28-43). */
+typedef struct qk_predicate {
+    int32_t op;                   /* QK_GREATER .. QK_NOT_EQUAL */
+    int32_t type;                 /* type of `column` */
+    const void* column;           /* device pointer, one element per row of the block */
+    const uint64_t* null_mask;    /* NULL = column not nullable */
+    int32_t rhs_kind;             /* QK_CONST or QK_COLUMN */
+    int32_t rhs_type;             /* type of the constant or of rhs_column */
+    const void* rhs_column;
+    const uint64_t* rhs_null_mask;
+    int64_t rhs_int;              /* constant when rhs_type is QK_INT8/QK_INT/QK_LONG */
+    double rhs_real;              /* constant when rhs_type is QK_FLOAT/QK_DOUBLE */
+} qk_predicate;
+
+typedef struct qk_groupby_s* qk_groupby;
+
+/* qk_gb_create flags */
+enum qk_gb_flags {
+    /* The caller promises that no key column is ever nullable: narrower key records. */
+    QK_GB_KEYS_NOT_NULL = 1u << 0,
+    /* ProcessBlock calls do not wait for the kernel; a HASH_TABLE_FULL latched by one of them
+     * is returned by qk_gb_check_error / qk_gb_get_results instead. */
+    QK_GB_ASYNC = 1u << 1,
+    /* Instead of failing with QK_HASH_TABLE_FULL the table grows itself (rows that found no
+     * slot are re-aggregated after the growth; nothing restarts). */
+    QK_GB_AUTO_GROW = 1u << 2
+};
+
+/* Create a hash table for GROUP BY over nkeys key columns (nkeys == 1: single key,
+ * GPUGroupBy<AGG,O,K,V>, GPUGroupBy.cuh:313-362; nkeys > 1: composite key,
+ * GPUGroupBy<AGG,O,std::vector<void*>,V>, GPUGroupByMultiKey.cuh:330-420).
+ * aggs[naggs] lists the aggregates kept per group (the reference keeps one per table; a
+ * table with several fuses them in one pass).  out_types[naggs] may be NULL for the
+ * reference typing (COUNT -> QK_LONG, others -> value_type) or widen SUM (QK_INT -> QK_LONG,
+ * QK_FLOAT -> QK_DOUBLE).  value_type is ignored when only COUNT/COUNT_ALL/NONE are kept.
+ * The table accepts max_hash_count * key_size_multiplier distinct non-NULL keys (capped at
+ * 2^30, IGroupBy.h:6) before it reports QK_HASH_TABLE_FULL, like the reference. */
+int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32_t value_type,
+                 int32_t naggs, const int32_t* aggs, const int32_t* out_types,
+                 int32_t max_hash_count, int32_t key_size_multiplier, uint32_t flags);
+int qk_gb_destroy(qk_groupby h);
+/* Drop every group; the table is then as freshly created. */
+int qk_gb_reset(qk_groupby h, void* stream);
+/* GPUGroupBy::GetMaxHashCount (GPUGroupBy.cuh:414-417): the key capacity. */
+int qk_gb_get_max_hash_count(qk_groupby h, int32_t* out);
+
+/* Aggregate one block of n rows.  keys[nkeys] are device columns; key_nulls is NULL or an
+ * array of nkeys masks (each may be NULL); values may be NULL when only COUNT/NONE are kept;
+ * value_nulls may be NULL.  Replaces GPUGroupBy::ProcessBlock (GPUGroupBy.cuh:395-410 and
+ * GPUGroupByMultiKey.cuh:535-600): unless the table is QK_GB_ASYNC the call waits and returns
+ * QK_HASH_TABLE_FULL like ErrorFlagSwapper::Swap (ErrorFlagSwapper.h:32-48). */
+int qk_gb_process_block(qk_groupby h, const void* const* keys, const uint64_t* const* key_nulls,
+                        const void* values, const uint64_t* value_nulls, int32_t n, void* stream);
+
+/* ProcessBlock restricted to rows whose int8 mask is non-zero: the same groups as compacting
+ * keys and values by the mask first (reconstructColKeep) and calling ProcessBlock on the
+ * result, which is the dispatcher's sequence (GpuSqlDispatcherAggregationFunctions.h:422-455,
+ * :532-536), without materialising the compacted columns. */
+int qk_gb_process_block_masked(qk_groupby h, const void* const* keys,
+                               const uint64_t* const* key_nulls, const void* values,
+                               const uint64_t* value_nulls, const int8_t* mask, int32_t n,
+                               void* stream);
+
+/* ProcessBlock restricted to rows that pass every one of npreds predicates (a conjunctive
+ * WHERE), evaluated in the same kernel.  Same groups as filter -> AND -> compact ->
+ * ProcessBlock in the reference (GpuSqlDispatcherBinaryFunctions.h:9-158,
+ * GpuSqlDispatcher.cpp:1107-1114, GpuSqlDispatcherAggregationFunctions.h:377-553). */
+int qk_gb_process_block_where(qk_groupby h, const void* const* keys,
+                              const uint64_t* const* key_nulls, const void* values,
+                              const uint64_t* value_nulls, int32_t n, int32_t npreds,
+                              const qk_predicate* preds, void* stream);
+
+/* Wait for the table's work and return its latched error (QK_SUCCESS or QK_HASH_TABLE_FULL),
+ * clearing it. */
+int qk_gb_check_error(qk_groupby h, void* stream);
+
+/* Final results: one row per group, in hash-slot order (unspecified, as in the reference).
+ * out_keys[nkeys] receive device columns of the key types; out_key_nulls (NULL or
+ * [nkeys]) receive key null bit masks; out_values[naggs] receive one device column per
+ * aggregate (NULL for QK_AGG_NONE); out_value_nulls (NULL or [naggs]) receive value null bit
+ * masks (bit set = no non-NULL value reached the group; COUNT never NULL).  All allocated
+ * with qk_malloc; zero groups give NULL pointers and *out_count == 0.  Replaces
+ * GPUGroupBy::GetResults (GPUGroupBy.cuh:478-587, GPUGroupByMultiKey.cuh:742-923). */
+int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls,
+                      void** out_values, uint64_t** out_value_nulls, int32_t* out_count,
+                      void* stream);
+
+/* Number of groups currently in the table (waits for the table's work). */
+int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream);
+
+/* Kernel timing: when enabled, HIP events bracket every row-aggregation kernel on its
+ * stream; *ms is the summed duration, *launches the number of launches measured. */
+int qk_gb_set_profiling(qk_groupby h, int32_t enable);
+int qk_gb_get_profile(qk_groupby h, double* ms, int64_t* launches);
+
+/* Describe the table's layout: bytes per global slot, number of slots, LDS slots per
+ * workgroup used by the last launch. */
+int qk_gb_describe(qk_groupby h, int32_t* slot_bytes, int64_t* slots, int32_t* lds_slots);
+
+/* Launch shape of the row kernel (no reference counterpart; the reference always launches
+ * ceil(n/1024) x maxThreadsPerBlock, Context.cu:109-127).  block: threads per workgroup
+ * (multiple of 64, <= 1024); lds_bytes: LDS table bytes per workgroup (0 = no LDS stage);
+ * wgs_per_cu: resident workgroups per CU (0 = as many as fit).  A negative argument leaves
+ * that setting unchanged. */
+int qk_gb_set_tuning(qk_groupby h, int32_t block, int32_t lds_bytes, int32_t wgs_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QIKKDB_AMD_H */

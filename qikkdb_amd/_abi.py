@@ -1,0 +1,145 @@
+"""ctypes binding of the C ABI in include/qikkdb_amd.h.
+
+The product path is the HIP library built in-tree (qikkdb_amd/libqikkdb_amd.so).  There is
+no CPU fallback: loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqikkdb_amd.so")
+
+# ---- constants (include/qikkdb_amd.h) -------------------------------------------------
+QK_SUCCESS, QK_ERROR, QK_DIVISION_BY_ZERO, QK_INTEGER_OVERFLOW = 0, 1, 2, 3
+QK_HASH_TABLE_FULL, QK_UNKNOWN_AGG_FUN = 4, 5
+QK_RUNTIME_ERROR, QK_OUT_OF_MEMORY = 64, 65
+
+INT, LONG, FLOAT, DOUBLE, INT8 = 8, 9, 10, 11, 15
+GREATER, GREATER_EQUAL, LESS, LESS_EQUAL, EQUAL, NOT_EQUAL, LOGICAL_AND, LOGICAL_OR = range(8)
+COLUMN, CONST = 0, 1
+AGG_MIN, AGG_MAX, AGG_SUM, AGG_AVG, AGG_COUNT, AGG_NONE, AGG_COUNT_ALL = range(7)
+GB_KEYS_NOT_NULL, GB_ASYNC, GB_AUTO_GROW = 1, 2, 4
+
+NP_OF = {INT: np.int32, LONG: np.int64, FLOAT: np.float32, DOUBLE: np.float64, INT8: np.int8}
+TYPE_OF = {np.dtype(v): k for k, v in NP_OF.items()}
+SIZE_OF = {INT: 4, LONG: 8, FLOAT: 4, DOUBLE: 8, INT8: 1}
+
+ERROR_NAMES = {
+    QK_ERROR: "GPU_EXTENSION_ERROR",
+    QK_DIVISION_BY_ZERO: "GPU_DIVISION_BY_ZERO_ERROR",
+    QK_INTEGER_OVERFLOW: "GPU_INTEGER_OVERFLOW_ERROR",
+    QK_HASH_TABLE_FULL: "GPU_HASH_TABLE_FULL",
+    QK_UNKNOWN_AGG_FUN: "GPU_UNKNOWN_AGG_FUN",
+    QK_RUNTIME_ERROR: "HIP_RUNTIME_ERROR",
+    QK_OUT_OF_MEMORY: "OUT_OF_MEMORY",
+}
+
+
+class QueryEngineError(RuntimeError):
+    """Mirror of the reference's query_engine_error (QueryEngine/GPUError.h:43-65)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {message}")
+        self.code = code
+
+    def GetQueryEngineError(self) -> int:  # noqa: N802 - reference method name
+        return self.code
+
+
+class Predicate(C.Structure):
+    """qk_predicate: one comparison of a conjunctive WHERE."""
+
+    _fields_ = [
+        ("op", C.c_int32),
+        ("type", C.c_int32),
+        ("column", C.c_void_p),
+        ("null_mask", C.c_void_p),
+        ("rhs_kind", C.c_int32),
+        ("rhs_type", C.c_int32),
+        ("rhs_column", C.c_void_p),
+        ("rhs_null_mask", C.c_void_p),
+        ("rhs_int", C.c_int64),
+        ("rhs_real", C.c_double),
+    ]
+
+
+_vp, _i32, _i64, _u32, _sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_size_t
+_P = C.POINTER
+
+# name -> argtypes (every function returns int status, except qk_last_error_message)
+PROTOTYPES = {
+    "qk_init": [C.c_int],
+    "qk_device_count": [_P(C.c_int)],
+    "qk_malloc": [_P(_vp), _sz],
+    "qk_free": [_vp],
+    "qk_mem_stats": [_P(_sz), _P(_sz)],
+    "qk_mem_release_cached": [],
+    "qk_memcpy_htod": [_vp, _vp, _sz, _vp],
+    "qk_memcpy_dtoh": [_vp, _vp, _sz, _vp],
+    "qk_memcpy_dtod": [_vp, _vp, _sz, _vp],
+    "qk_memset": [_vp, C.c_int, _sz, _vp],
+    "qk_stream_create": [_P(_vp)],
+    "qk_stream_destroy": [_vp],
+    "qk_stream_synchronize": [_vp],
+    "qk_device_synchronize": [],
+    "qk_filter": [_i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
+    "qk_filter_bits": [_i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
+    "qk_logical_not": [_vp, _i32, _vp, _vp, _vp],
+    "qk_prefix_sum": [_vp, _i32, _vp, _i32, _vp],
+    "qk_mask_sum": [_vp, _i32, _P(_i64), _vp],
+    "qk_reconstruct_col_keep": [_i32, _vp, _vp, _i32, _vp, _P(_vp), _P(_i32), _P(_vp), _vp],
+    "qk_generate_indexes_keep": [_vp, _i32, _i32, _P(_vp), _P(_i32), _vp],
+    "qk_compress_null_mask": [_vp, _i32, _P(_vp), _vp],
+    "qk_where_compact": [_i32, _P(Predicate), _i32, _P(_i32), _P(_vp), _P(_vp), _i32, _P(_vp),
+                         _P(_vp), _P(_i32), _vp],
+    "qk_gb_create": [_P(_vp), _i32, _P(_i32), _i32, _i32, _P(_i32), _P(_i32), _i32, _i32, _u32],
+    "qk_gb_destroy": [_vp],
+    "qk_gb_reset": [_vp, _vp],
+    "qk_gb_get_max_hash_count": [_vp, _P(_i32)],
+    "qk_gb_process_block": [_vp, _P(_vp), _P(_vp), _vp, _vp, _i32, _vp],
+    "qk_gb_process_block_masked": [_vp, _P(_vp), _P(_vp), _vp, _vp, _vp, _i32, _vp],
+    "qk_gb_process_block_where": [_vp, _P(_vp), _P(_vp), _vp, _vp, _i32, _i32, _P(Predicate), _vp],
+    "qk_gb_check_error": [_vp, _vp],
+    "qk_gb_get_results": [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_i32), _vp],
+    "qk_gb_group_count": [_vp, _P(_i64), _vp],
+    "qk_gb_set_profiling": [_vp, _i32],
+    "qk_gb_get_profile": [_vp, _P(C.c_double), _P(_i64)],
+    "qk_gb_describe": [_vp, _P(_i32), _P(_i64), _P(_i32)],
+    "qk_gb_set_tuning": [_vp, _i32, _i32, _i32],
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build the HIP extension first "
+            "(python -m qikkdb_amd.build or __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, args in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.qk_last_error_message.argtypes = []
+    lib.qk_last_error_message.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != QK_SUCCESS:
+        msg = _lib.qk_last_error_message().decode(errors="replace") if _lib else ""
+        raise QueryEngineError(status, msg)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))

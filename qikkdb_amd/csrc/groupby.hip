@@ -1,0 +1,1201 @@
+// groupby.hip - hash GROUP BY with a fused WHERE, for gfx950.
+//
+// Counterpart of GPUGroupBy<AGG,O,K,V> (QueryEngine/GPUCore/GPUGroupBy.cuh:68-791) and its
+// composite-key specialisation (GPUGroupByMultiKey.cuh:156-1325).  Design differences:
+//  * One table keeps several aggregates per group (the reference keeps one, S11).
+//  * Every workgroup first aggregates into its own open-addressing table in LDS (bounded
+//    linear probing, native LDS atomics); a row whose key finds no LDS slot goes straight to
+//    the global table, and the LDS table is merged into the global one when the workgroup
+//    ends.  The reference instead spreads each slot over 256 replicas in HBM
+//    (IGroupBy.h:5, GPUGroupBy.cuh:181-191) and folds them in GetResults.
+//  * Global slots are records (key words + counters + accumulators) so one probe touches one
+//    line; single-word keys are claimed with a 64-bit CAS, multi-word keys by claiming the
+//    last word (empty -> busy -> key).
+//  * MIN/MAX of every type use native unsigned 64-bit atomics on an order-preserving
+//    encoding; SUM of FLOAT/DOUBLE accumulates in FP64, integer SUM in wrapping int64.
+//  * Results come out in slot order through a look-back compaction, in one kernel.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "groupby.hpp"
+#include "lookback.hpp"
+
+namespace qk {
+
+struct NoValue {};
+
+namespace {
+
+constexpr int kSteps = 4;        // rows per thread per tile
+constexpr int kLdsProbes = 8;    // LDS probe limit before a row goes to the global table
+constexpr int kEmitBlock = 256;
+constexpr int kEmitSteps = 16;
+constexpr int kEmitTile = kEmitBlock * kEmitSteps;
+
+// ---- key normalisation -----------------------------------------------------------------
+
+// Bits of a FLOAT/DOUBLE key with -0.0 folded into +0.0 (so equal values group together).
+__device__ __forceinline__ uint64_t fkey32(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x7fffffffu) ? (uint64_t)b : 0ull;
+}
+__device__ __forceinline__ uint64_t fkey64(double d) {
+    uint64_t b = (uint64_t)__double_as_longlong(d);
+    return (b & 0x7fffffffffffffffull) ? b : 0ull;
+}
+
+// Single key column widened to one 64-bit word.
+__device__ __forceinline__ uint64_t key_single(int type, const void* col, int64_t r) {
+    switch (type) {
+    case QK_INT: return (uint64_t)(int64_t)((const int32_t*)col)[r];
+    case QK_LONG: return (uint64_t)((const int64_t*)col)[r];
+    case QK_FLOAT: return fkey32(((const float*)col)[r]);
+    case QK_DOUBLE: return fkey64(((const double*)col)[r]);
+    default: return (uint64_t)(int64_t)((const int8_t*)col)[r];
+    }
+}
+
+// Field bits of a composite key column: 32-bit fields for INT/FLOAT/INT8, 64-bit otherwise.
+__device__ __forceinline__ uint64_t key_field(int type, const void* col, int64_t r) {
+    switch (type) {
+    case QK_INT: return (uint64_t)(uint32_t)((const int32_t*)col)[r];
+    case QK_LONG: return (uint64_t)((const int64_t*)col)[r];
+    case QK_FLOAT: return fkey32(((const float*)col)[r]);
+    case QK_DOUBLE: return fkey64(((const double*)col)[r]);
+    default: return (uint64_t)(uint32_t)(int32_t)((const int8_t*)col)[r];
+    }
+}
+
+template <int KW>
+__device__ __forceinline__ void load_key(const KeySpec& ks, const KeyCols& kc, int64_t r,
+                                         uint64_t (&k)[KW], bool& knull) {
+    knull = false;
+    if (KW == 1 && ks.single) {
+        knull = kc.nm[0] && null_bit(kc.nm[0], r);
+        k[0] = knull ? 0ull : key_single(ks.type[0], kc.col[0], r);
+        return;
+    }
+#pragma unroll
+    for (int w = 0; w < KW; w++) k[w] = 0;
+    for (int c = 0; c < ks.nkeys; c++) {
+        const bool isnull = kc.nm[c] && null_bit(kc.nm[c], r);
+        if (isnull) {
+            if (KW > 1) {
+#pragma unroll
+                for (int w = 0; w < KW; w++)
+                    if (w == ks.nword[c]) k[w] |= 1ull << ks.nshift[c];
+            }
+        } else {
+            const uint64_t f = key_field(ks.type[c], kc.col[c], r) << ks.shift[c];
+#pragma unroll
+            for (int w = 0; w < KW; w++)
+                if (w == ks.word[c]) k[w] |= f;
+        }
+    }
+    if (KW > 1) k[KW - 1] |= kPresent;
+}
+
+template <int KW>
+__device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[KW]) {
+    uint64_t h = hash64(k[0]);
+#pragma unroll
+    for (int w = 1; w < KW; w++) h = hash64(h ^ (k[w] + 0x9e3779b97f4a7c15ull * (uint64_t)w));
+    return h;
+}
+
+// ---- value handling ----------------------------------------------------------------------
+
+template <typename VT> struct VTraits {
+    static constexpr bool has = true;
+    static constexpr bool real = std::is_floating_point<VT>::value;
+};
+template <> struct VTraits<NoValue> {
+    static constexpr bool has = false;
+    static constexpr bool real = false;
+};
+
+// Partial aggregate of one row or one LDS slot.
+struct Part {
+    uint64_t cnt, cntall, sum, mn, mx;  // sum holds FP64 bits or wrapping int64
+};
+
+// ---- global table --------------------------------------------------------------------
+
+__device__ __forceinline__ void g_new_key(const GView& g) {
+    unsigned long long c = atomicAdd(g.nkeys, 1ull) + 1ull;
+    if (c > g.logical_cap)
+        __hip_atomic_store(g.err, (int32_t)QK_HASH_TABLE_FULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void g_claim_side(const GView& g, uint64_t s, bool counts) {
+    uint64_t* kp = rec_ptr(g, s);
+    if (*(volatile uint64_t*)kp == 0ull) {
+        unsigned long long old = atomicCAS((unsigned long long*)kp, 0ull, 1ull);
+        if (old == 0ull && counts) g_new_key(g);
+    }
+}
+
+// Slot of a single-word key (inserting it when absent); -1 when the table has no free slot.
+__device__ __forceinline__ int64_t g_slot1(const GView& g, uint64_t key, bool knull, uint64_t h) {
+    if (knull) {
+        g_claim_side(g, g.cap, false);
+        return (int64_t)g.cap;
+    }
+    if (key == kEmpty) {
+        g_claim_side(g, g.cap + 1, true);
+        return (int64_t)(g.cap + 1);
+    }
+    const uint64_t m = g.cap - 1;
+    uint64_t s = h & m;
+    for (uint64_t p = 0; p < g.cap; p++) {
+        uint64_t* kp = rec_ptr(g, s);
+        uint64_t cur = *(volatile uint64_t*)kp;  // a stale kEmpty only costs a CAS
+        if (cur == key) return (int64_t)s;
+        if (cur == kEmpty) {
+            unsigned long long old = atomicCAS((unsigned long long*)kp, (unsigned long long)kEmpty,
+                                               (unsigned long long)key);
+            if (old == kEmpty) {
+                g_new_key(g);
+                return (int64_t)s;
+            }
+            if (old == key) return (int64_t)s;
+        }
+        s = (s + 1) & m;
+    }
+    return -1;
+}
+
+// Slot of a multi-word key: the last word is claimed (0 -> kBusy), the other words written,
+// then the last word published with release order; readers that see kBusy wait for it.
+template <int KW>
+__device__ __forceinline__ int64_t g_slotN(const GView& g, const uint64_t (&k)[KW], uint64_t h) {
+    const uint64_t m = g.cap - 1;
+    uint64_t s = h & m;
+    for (uint64_t p = 0; p < g.cap; p++) {
+        uint64_t* kp = rec_ptr(g, s);
+        uint64_t last = __hip_atomic_load(kp + (KW - 1), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (last == 0ull) {
+            unsigned long long old = atomicCAS((unsigned long long*)(kp + (KW - 1)), 0ull,
+                                               (unsigned long long)kBusy);
+            if (old == 0ull) {
+#pragma unroll
+                for (int w = 0; w < KW - 1; w++) st_relaxed(kp + w, k[w]);
+                __hip_atomic_store(kp + (KW - 1), k[KW - 1], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                g_new_key(g);
+                return (int64_t)s;
+            }
+            last = old;
+        }
+        while (last == kBusy) {
+            __builtin_amdgcn_s_sleep(1);
+            last = __hip_atomic_load(kp + (KW - 1), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (last == k[KW - 1]) {
+            bool eq = true;
+#pragma unroll
+            for (int w = 0; w < KW - 1; w++) eq = eq && (ld_relaxed(kp + w) == k[w]);
+            if (eq) return (int64_t)s;
+        }
+        s = (s + 1) & m;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void g_accum(const GView& g, int64_t s, const Part& p, uint32_t fields) {
+    uint64_t* r = rec_ptr(g, (uint64_t)s);
+    if (p.cnt) atomicAdd((unsigned long long*)(r + g.rec.cnt), (unsigned long long)p.cnt);
+    if ((fields & F_CNTALL) && p.cntall)
+        atomicAdd((unsigned long long*)(r + g.rec.cntall), (unsigned long long)p.cntall);
+    if (p.cnt) {
+        if (fields & F_SUM) {
+            if (g.real_sum)
+                atomicAdd((double*)(r + g.rec.sum), __longlong_as_double((long long)p.sum));
+            else
+                atomicAdd((unsigned long long*)(r + g.rec.sum), (unsigned long long)p.sum);
+        }
+        if (fields & F_MIN) atomicMin((unsigned long long*)(r + g.rec.mn), (unsigned long long)p.mn);
+        if (fields & F_MAX) atomicMax((unsigned long long*)(r + g.rec.mx), (unsigned long long)p.mx);
+    }
+}
+
+__device__ __forceinline__ void g_fail(const GView& g) {
+    __hip_atomic_store(g.err, (int32_t)QK_HASH_TABLE_FULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- LDS table ---------------------------------------------------------------------------
+// Arrays of H hashed slots + 2 side slots (NULL key, key == kEmpty).
+struct Lds {
+    uint64_t* key;
+    uint64_t* sum;
+    uint64_t* mn;
+    uint64_t* mx;
+    uint32_t* cnt;
+    uint32_t* cntall;
+    int32_t H;
+};
+
+__host__ __device__ __forceinline__ int lds_slot_bytes(uint32_t fields) {
+    return 8 + 4 + ((fields & F_CNTALL) ? 4 : 0) + ((fields & F_SUM) ? 8 : 0) +
+           ((fields & F_MIN) ? 8 : 0) + ((fields & F_MAX) ? 8 : 0);
+}
+
+__device__ __forceinline__ Lds lds_carve(void* base, int32_t H, uint32_t fields) {
+    Lds t;
+    const int T = H + 2;
+    char* p = (char*)base;
+    t.key = (uint64_t*)p; p += (size_t)T * 8;
+    t.sum = (fields & F_SUM) ? (uint64_t*)p : nullptr; p += (fields & F_SUM) ? (size_t)T * 8 : 0;
+    t.mn = (fields & F_MIN) ? (uint64_t*)p : nullptr; p += (fields & F_MIN) ? (size_t)T * 8 : 0;
+    t.mx = (fields & F_MAX) ? (uint64_t*)p : nullptr; p += (fields & F_MAX) ? (size_t)T * 8 : 0;
+    t.cnt = (uint32_t*)p; p += (size_t)T * 4;
+    t.cntall = (fields & F_CNTALL) ? (uint32_t*)p : nullptr;
+    t.H = H;
+    return t;
+}
+
+__device__ __forceinline__ int lds_find(const Lds& t, uint64_t key, bool knull, uint64_t h) {
+    if (knull) {
+        if (t.key[t.H] == 0ull) t.key[t.H] = 1ull;
+        return t.H;
+    }
+    if (key == kEmpty) {
+        if (t.key[t.H + 1] == 0ull) t.key[t.H + 1] = 1ull;
+        return t.H + 1;
+    }
+    // range reduction of the high hash bits onto [0, H)
+    uint32_t s = (uint32_t)(((h >> 32) * (uint64_t)(uint32_t)t.H) >> 32);
+#pragma unroll 1
+    for (int p = 0; p < kLdsProbes; p++) {
+        uint64_t cur = __hip_atomic_load(&t.key[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == key) return (int)s;
+        if (cur == kEmpty) {
+            unsigned long long old = atomicCAS((unsigned long long*)&t.key[s], (unsigned long long)kEmpty,
+                                               (unsigned long long)key);
+            if (old == kEmpty || old == key) return (int)s;
+        }
+        s = (s + 1 == (uint32_t)t.H) ? 0u : s + 1;
+    }
+    return -1;
+}
+
+template <bool REAL>
+__device__ __forceinline__ void lds_accum(const Lds& t, int s, const Part& p, uint32_t fields) {
+    if (p.cnt) atomicAdd(&t.cnt[s], (uint32_t)p.cnt);
+    if (fields & F_CNTALL) atomicAdd(&t.cntall[s], (uint32_t)p.cntall);
+    if (p.cnt) {
+        if (fields & F_SUM) {
+            if (REAL)
+                atomicAdd((double*)&t.sum[s], __longlong_as_double((long long)p.sum));
+            else
+                atomicAdd((unsigned long long*)&t.sum[s], (unsigned long long)p.sum);
+        }
+        if (fields & F_MIN) atomicMin((unsigned long long*)&t.mn[s], (unsigned long long)p.mn);
+        if (fields & F_MAX) atomicMax((unsigned long long*)&t.mx[s], (unsigned long long)p.mx);
+    }
+}
+
+// ---- the row kernel ----------------------------------------------------------------------
+// MODE 0: every row; 1: rows whose int8 mask is non-zero; 2: rows passing the predicates.
+// Row r of tile T is T*(kSteps*blockDim) + k*blockDim + threadIdx.x, so each step's loads
+// are coalesced spans.  Workgroups are persistent (grid-stride over tiles) so that the LDS
+// table sees many rows before it is merged.
+template <int KW, typename VT, int MODE>
+__global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols kc,
+                                                  const VT* __restrict__ vals,
+                                                  const uint64_t* __restrict__ vnull,
+                                                  const int8_t* __restrict__ mask, PredSet preds,
+                                                  uint32_t vpred, int64_t n, uint32_t fields,
+                                                  int32_t H) {
+    extern __shared__ uint64_t lds_raw[];
+    constexpr bool HAS_V = VTraits<VT>::has;
+    constexpr bool REAL = VTraits<VT>::real;
+    const bool use_lds = (KW == 1) && H > 0;
+    Lds t{};
+    if (use_lds) {
+        t = lds_carve(lds_raw, H, fields);
+        for (int s = threadIdx.x; s < H + 2; s += blockDim.x) {
+            t.key[s] = s < H ? kEmpty : 0ull;
+            t.cnt[s] = 0;
+            if (t.cntall) t.cntall[s] = 0;
+            if (t.sum) t.sum[s] = 0;
+            if (t.mn) t.mn[s] = ~0ull;
+            if (t.mx) t.mx[s] = 0ull;
+        }
+        __syncthreads();
+    }
+
+    const int64_t bd = blockDim.x;
+    const int64_t tile_rows = bd * kSteps;
+    const int64_t ntiles = (n + tile_rows - 1) / tile_rows;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t key[kSteps][KW];
+        bool kn[kSteps], act[kSteps], vn[kSteps];
+        typename std::conditional<HAS_V, VT, int>::type val[kSteps];
+#pragma unroll
+        for (int k = 0; k < kSteps; k++) {
+            const int64_t r = tile * tile_rows + k * bd + threadIdx.x;
+            act[k] = r < n;
+            vn[k] = false;
+            kn[k] = false;
+            if (act[k]) {
+                load_key<KW>(ks, kc, r, key[k], kn[k]);
+                if constexpr (HAS_V) {
+                    val[k] = vals[r];
+                } else {
+                    val[k] = 0;
+                }
+                vn[k] = vnull && null_bit(vnull, r);
+                if (MODE == 1) act[k] = mask[r] != 0;
+            }
+        }
+        if (MODE == 2) {
+#pragma unroll
+            for (int k = 0; k < kSteps; k++) {
+                if (!act[k]) continue;
+                const int64_t r = tile * tile_rows + k * bd + threadIdx.x;
+                bool pass = true;
+                for (int p = 0; p < preds.n && pass; p++) {
+                    const qk_predicate& q = preds.p[p];
+                    if constexpr (HAS_V) {
+                        if ((vpred >> p) & 1u) {
+                            pass = !vn[k] && pred_on_value(q, val[k], r);
+                            continue;
+                        }
+                    }
+                    pass = eval_pred(q, r);
+                }
+                act[k] = pass;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kSteps; k++) {
+            if (!act[k]) continue;
+            Part p;
+            p.cntall = 1;
+            p.cnt = vn[k] ? 0 : 1;
+            p.sum = 0;
+            p.mn = ~0ull;
+            p.mx = 0;
+            if constexpr (HAS_V) {
+                if (!vn[k]) {
+                    if constexpr (REAL) {
+                        const double d = (double)val[k];
+                        p.sum = (uint64_t)__double_as_longlong(d);
+                        p.mn = p.mx = enc_f64(d);
+                    } else {
+                        const int64_t iv = (int64_t)val[k];
+                        p.sum = (uint64_t)iv;
+                        p.mn = p.mx = enc_i64(iv);
+                    }
+                }
+            }
+            const uint64_t h = key_hash<KW>(key[k]);
+            if constexpr (KW == 1) {
+                int ls = use_lds ? lds_find(t, key[k][0], kn[k], h) : -1;
+                if (ls >= 0) {
+                    lds_accum<REAL>(t, ls, p, fields);
+                } else {
+                    const int64_t gs = g_slot1(g, key[k][0], kn[k], h);
+                    if (gs >= 0) g_accum(g, gs, p, fields); else g_fail(g);
+                }
+            } else {
+                const int64_t gs = g_slotN<KW>(g, key[k], h);
+                if (gs >= 0) g_accum(g, gs, p, fields); else g_fail(g);
+            }
+        }
+    }
+
+    if (use_lds) {
+        __syncthreads();
+        for (int s = threadIdx.x; s < H + 2; s += blockDim.x) {
+            const uint64_t k = t.key[s];
+            if (s < H ? (k == kEmpty) : (k == 0ull)) continue;
+            Part p;
+            p.cnt = t.cnt[s];
+            p.cntall = t.cntall ? t.cntall[s] : 0;
+            p.sum = t.sum ? t.sum[s] : 0;
+            p.mn = t.mn ? t.mn[s] : ~0ull;
+            p.mx = t.mx ? t.mx[s] : 0ull;
+            int64_t gs;
+            if (s == H) {
+                gs = g_slot1(g, 0ull, true, 0);
+            } else if (s == H + 1) {
+                gs = g_slot1(g, kEmpty, false, 0);
+            } else {
+                gs = g_slot1(g, k, false, hash64(k));
+            }
+            if (gs >= 0) g_accum(g, gs, p, fields); else g_fail(g);
+        }
+    }
+}
+
+// ---- table initialisation ----------------------------------------------------------------
+__global__ void k_gb_init(GView g, int32_t kw) {
+    const uint64_t total = g.cap + 2;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += stride) {
+        uint64_t* r = rec_ptr(g, s);
+        for (int w = 0; w < kw; w++) r[w] = (kw == 1 && s < g.cap) ? kEmpty : 0ull;
+        r[g.rec.cnt] = 0;
+        if (g.rec.cntall >= 0) r[g.rec.cntall] = 0;
+        if (g.rec.sum >= 0) r[g.rec.sum] = 0;
+        if (g.rec.mn >= 0) r[g.rec.mn] = ~0ull;
+        if (g.rec.mx >= 0) r[g.rec.mx] = 0;
+    }
+}
+
+// ---- results ------------------------------------------------------------------------------
+struct OutSpec {
+    int32_t naggs;
+    int32_t vtype;
+    int32_t agg[kMaxAggs];
+    int32_t otype[kMaxAggs];
+    void* out[kMaxAggs];
+    uint64_t* onull[kMaxAggs];
+    void* kout[kMaxKeys];
+    uint64_t* konull[kMaxKeys];
+};
+
+__device__ __forceinline__ bool slot_used(const GView& g, int32_t kw, uint64_t s) {
+    const uint64_t* r = rec_ptr(g, s);
+    if (kw == 1) return s < g.cap ? (r[0] != kEmpty) : (r[0] != 0ull);
+    return s < g.cap && r[kw - 1] != 0ull;
+}
+
+__device__ __forceinline__ void put_typed(void* out, int type, int64_t pos, int64_t iv, double dv,
+                                          bool real) {
+    switch (type) {
+    case QK_INT8: ((int8_t*)out)[pos] = (int8_t)(real ? (int64_t)dv : iv); break;
+    case QK_INT: ((int32_t*)out)[pos] = (int32_t)(real ? (int64_t)dv : iv); break;
+    case QK_LONG: ((int64_t*)out)[pos] = real ? (int64_t)dv : iv; break;
+    case QK_FLOAT: ((float*)out)[pos] = real ? (float)dv : (float)iv; break;
+    default: ((double*)out)[pos] = real ? dv : (double)iv; break;
+    }
+}
+
+__device__ __forceinline__ void set_bit(uint64_t* m, int64_t pos) {
+    atomicOr((unsigned long long*)(m + (pos >> 6)), 1ull << (pos & 63));
+}
+
+// Key column c of the group in slot s.
+__device__ __forceinline__ void emit_key(const GView& g, const KeySpec& ks, const OutSpec& os,
+                                         uint64_t s, int64_t pos) {
+    const uint64_t* r = rec_ptr(g, s);
+    for (int c = 0; c < ks.nkeys; c++) {
+        const int type = ks.type[c];
+        bool isnull = false;
+        uint64_t v;
+        if (ks.kw == 1) {
+            const uint64_t w = (s == g.cap) ? 0ull : (s == g.cap + 1 ? kEmpty : r[0]);
+            if (ks.single) {
+                isnull = (s == g.cap);
+                v = w;
+            } else {
+                v = w >> ks.shift[c];
+            }
+        } else {
+            isnull = ks.nullable && ((r[ks.nword[c]] >> ks.nshift[c]) & 1ull);
+            v = r[ks.word[c]] >> ks.shift[c];
+        }
+        if (isnull) v = 0;
+        void* o = os.kout[c];
+        if (o) {
+            switch (type) {
+            case QK_INT: ((int32_t*)o)[pos] = (int32_t)(uint32_t)v; break;
+            case QK_LONG: ((int64_t*)o)[pos] = (int64_t)v; break;
+            case QK_FLOAT: ((uint32_t*)o)[pos] = (uint32_t)v; break;
+            case QK_DOUBLE: ((uint64_t*)o)[pos] = v; break;
+            default: ((int8_t*)o)[pos] = (int8_t)(uint32_t)v; break;
+            }
+        }
+        if (isnull && os.konull[c]) set_bit(os.konull[c], pos);
+    }
+}
+
+__device__ __forceinline__ void emit_aggs(const GView& g, const OutSpec& os, uint64_t s, int64_t pos) {
+    const uint64_t* r = rec_ptr(g, s);
+    const uint64_t cnt = r[g.rec.cnt];
+    const int vt = os.vtype;
+    const bool vreal = vt == QK_FLOAT || vt == QK_DOUBLE;
+    for (int a = 0; a < os.naggs; a++) {
+        const int ag = os.agg[a];
+        void* o = os.out[a];
+        if (ag == QK_AGG_NONE || !o) continue;
+        const int ot = os.otype[a];
+        bool isnull = false;
+        switch (ag) {
+        case QK_AGG_COUNT: ((int64_t*)o)[pos] = (int64_t)cnt; break;
+        case QK_AGG_COUNT_ALL: ((int64_t*)o)[pos] = (int64_t)r[g.rec.cntall]; break;
+        case QK_AGG_SUM: {
+            isnull = cnt == 0;
+            if (vreal) put_typed(o, ot, pos, 0, isnull ? 0.0 : __longlong_as_double((long long)r[g.rec.sum]), true);
+            else put_typed(o, ot, pos, isnull ? 0 : (int64_t)r[g.rec.sum], 0.0, false);
+            break;
+        }
+        case QK_AGG_MIN:
+        case QK_AGG_MAX: {
+            isnull = cnt == 0;
+            const uint64_t e = r[ag == QK_AGG_MIN ? g.rec.mn : g.rec.mx];
+            if (vreal) {
+                double d = isnull ? (ag == QK_AGG_MIN ? (vt == QK_FLOAT ? 3.4028234663852886e38 : 1.7976931348623157e308)
+                                                      : (vt == QK_FLOAT ? -3.4028234663852886e38 : -1.7976931348623157e308))
+                                  : dec_f64(e);
+                put_typed(o, ot, pos, 0, d, true);
+            } else {
+                int64_t lim_hi = vt == QK_INT ? INT32_MAX : (vt == QK_INT8 ? INT8_MAX : INT64_MAX);
+                int64_t lim_lo = vt == QK_INT ? INT32_MIN : (vt == QK_INT8 ? INT8_MIN : INT64_MIN);
+                int64_t v = isnull ? (ag == QK_AGG_MIN ? lim_hi : lim_lo) : dec_i64(e);
+                put_typed(o, ot, pos, v, 0.0, false);
+            }
+            break;
+        }
+        case QK_AGG_AVG: {
+            // sum / count with the sum in the value type (GPUGroupBy.cuh:536-558,
+            // GPUBinary.cuh:126-143); count 0 gives the NULL constant (NullConstants.cuh:8-26)
+            isnull = cnt == 0;
+            if (vreal) {
+                double d = isnull ? __longlong_as_double(0x7ff8000000000000ll)
+                                  : __longlong_as_double((long long)r[g.rec.sum]) / (double)cnt;
+                put_typed(o, ot, pos, 0, d, true);
+            } else {
+                int64_t v;
+                const uint64_t sm = r[g.rec.sum];
+                if (vt == QK_INT) v = isnull ? INT32_MIN : (int64_t)(int32_t)((int64_t)(int32_t)(uint32_t)sm / (int64_t)cnt);
+                else if (vt == QK_INT8) v = isnull ? INT8_MIN : (int64_t)(int8_t)((int64_t)(int8_t)(uint8_t)sm / (int64_t)cnt);
+                else v = isnull ? INT64_MIN : (int64_t)sm / (int64_t)cnt;
+                put_typed(o, ot, pos, v, 0.0, false);
+            }
+            break;
+        }
+        default: break;
+        }
+        if (isnull && os.onull[a]) set_bit(os.onull[a], pos);
+    }
+}
+
+__global__ void __launch_bounds__(kEmitBlock) k_gb_emit(GView g, KeySpec ks, OutSpec os,
+                                                        uint64_t* status, uint32_t* ticket,
+                                                        uint32_t epoch, int64_t tiles,
+                                                        int32_t* total, int64_t limit) {
+    constexpr int W = kEmitBlock / 64;
+    __shared__ int64_t s_tile;
+    __shared__ int32_t s_cnt[kEmitSteps][W];
+    __shared__ uint32_t s_excl;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) s_tile = lb_ticket(ticket, tiles);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const uint64_t nslots = g.cap + 2;
+    const uint64_t lt_mask = (1ull << lane) - 1;
+    uint32_t keep = 0, pre[kEmitSteps];
+#pragma unroll
+    for (int k = 0; k < kEmitSteps; k++) {
+        const uint64_t s = (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid;
+        const bool sel = s < nslots && slot_used(g, ks.kw, s);
+        const uint64_t b = __ballot(sel);
+        pre[k] = (uint32_t)__popcll(b & lt_mask);
+        if (lane == 0) s_cnt[k][wid] = (int32_t)__popcll(b);
+        keep |= (sel ? 1u : 0u) << k;
+    }
+    __syncthreads();
+    int32_t agg = 0;
+    for (int k = 0; k < kEmitSteps; k++)
+        for (int w = 0; w < W; w++) agg += s_cnt[k][w];
+    if (wid == 0) {
+        uint32_t e = lb_exclusive(status, epoch, tile, (uint32_t)agg);
+        if (lane == 0) {
+            s_excl = e;
+            if (tile == tiles - 1) *total = (int32_t)(e + (uint32_t)agg);
+        }
+    }
+    __syncthreads();
+    int64_t step_base = s_excl;
+    for (int k = 0; k < kEmitSteps; k++) {
+        int64_t wave_base = step_base;
+        for (int w = 0; w < wid; w++) wave_base += s_cnt[k][w];
+        if ((keep >> k) & 1u) {
+            const uint64_t s = (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid;
+            const int64_t pos = wave_base + pre[k];
+            if (pos < limit) {  // outputs hold `limit` rows; the host checks the total
+                emit_key(g, ks, os, s, pos);
+                emit_aggs(g, os, s, pos);
+            }
+        }
+        int32_t st = 0;
+        for (int w = 0; w < W; w++) st += s_cnt[k][w];
+        step_base += st;
+    }
+}
+
+// ---- launch helpers -------------------------------------------------------------------------
+
+template <int KW, typename VT, int MODE>
+int launch_rows_t(const GView& g, const KeySpec& ks, const KeyCols& kc, const void* vals,
+                  const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
+                  int64_t n, uint32_t fields, int32_t H, int block, int wgs_per_cu,
+                  hipStream_t s) {
+    auto kern = k_gb_rows<KW, VT, MODE>;
+    const size_t lds = (KW == 1 && H > 0) ? (size_t)(H + 2) * lds_slot_bytes(fields) : 0;
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)device_info().lds_per_block);
+    });
+    int occ = 0;
+    QK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds));
+    if (occ < 1) return fail(QK_ERROR, "group by: launch shape (block %d, LDS %zu B) does not fit", block, lds);
+    if (wgs_per_cu > 0 && wgs_per_cu < occ) occ = wgs_per_cu;
+    const int64_t tile_rows = (int64_t)block * kSteps;
+    const int64_t ntiles = (n + tile_rows - 1) / tile_rows;
+    int64_t grid = (int64_t)device_info().cus * occ;
+    if (grid > ntiles) grid = ntiles;
+    if (grid < 1) grid = 1;
+    kern<<<(unsigned)grid, block, lds, s>>>(g, ks, kc, (const VT*)vals, vnull, mask, ps, vpred, n,
+                                            fields, (KW == 1) ? H : 0);
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+template <int KW, typename VT>
+int launch_rows_m(int mode, const GView& g, const KeySpec& ks, const KeyCols& kc, const void* vals,
+                  const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
+                  int64_t n, uint32_t fields, int32_t H, int block, int wpc, hipStream_t s) {
+    switch (mode) {
+    case 0: return launch_rows_t<KW, VT, 0>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    case 1: return launch_rows_t<KW, VT, 1>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    default: return launch_rows_t<KW, VT, 2>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    }
+}
+
+template <int KW>
+int launch_rows_v(int vtype, int mode, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                  const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
+                  uint32_t vpred, int64_t n, uint32_t fields, int32_t H, int block, int wpc,
+                  hipStream_t s) {
+    if (!vals) return launch_rows_m<KW, NoValue>(mode, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    switch (vtype) {
+    case QK_INT: return launch_rows_m<KW, int32_t>(mode, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    case QK_LONG: return launch_rows_m<KW, int64_t>(mode, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    case QK_FLOAT: return launch_rows_m<KW, float>(mode, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    case QK_DOUBLE: return launch_rows_m<KW, double>(mode, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    default: return fail(QK_ERROR, "group by: unsupported value type %d", vtype);
+    }
+}
+
+}  // namespace
+}  // namespace qk
+
+using namespace qk;
+
+// ---- the table object ----------------------------------------------------------------------
+struct qk_groupby_s {
+    int device = 0;
+    int32_t nkeys = 0, key_types[kMaxKeys] = {}, value_type = 0, naggs = 0;
+    int32_t aggs[kMaxAggs] = {}, out_types[kMaxAggs] = {};
+    uint32_t flags = 0, fields = 0;
+    KeySpec ks{};
+    Rec rec{};
+    uint64_t logical_cap = 0;
+    int32_t reported_max = 0;
+    uint64_t cap = 0;
+    uint64_t* slots = nullptr;
+    TableCounters* ctr = nullptr;
+    bool real_sum = false;
+    // launch tuning
+    int32_t block = 512;
+    int32_t lds_budget = -1;  // bytes of LDS per workgroup; -1 = half of a CU's LDS, 0 = none
+    int32_t wgs_per_cu = 0;  // 0 = as many as fit
+    int32_t last_H = 0;
+    // profiling
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    size_t events_used = 0;
+    double prof_ms = 0.0;
+    int64_t prof_launches = 0;
+};
+
+namespace qk {
+namespace {
+
+GView view_of(const qk_groupby_s* h) {
+    GView g;
+    g.slots = h->slots;
+    g.cap = h->cap;
+    g.rec = h->rec;
+    g.nkeys = &h->ctr->nkeys;
+    g.err = &h->ctr->err;
+    g.logical_cap = h->logical_cap;
+    g.real_sum = h->real_sum ? 1 : 0;
+    return g;
+}
+
+bool agg_needs_value(int a) {
+    return a == QK_AGG_MIN || a == QK_AGG_MAX || a == QK_AGG_SUM || a == QK_AGG_AVG;
+}
+
+int init_table(qk_groupby_s* h, hipStream_t s) {
+    GView g = view_of(h);
+    const uint64_t total = h->cap + 2;
+    int64_t grid = (int64_t)((total + 255) / 256);
+    const int64_t capg = (int64_t)device_info().cus * 16;
+    if (grid > capg) grid = capg;
+    k_gb_init<<<(unsigned)grid, 256, 0, s>>>(g, h->ks.kw);
+    QK_HIP(hipGetLastError());
+    QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
+    return QK_SUCCESS;
+}
+
+int read_counters(qk_groupby_s* h, hipStream_t s, TableCounters* out) {
+    void* hb = nullptr;
+    QK_TRY(pinned_host(sizeof(TableCounters), &hb));
+    QK_HIP(hipMemcpyAsync(hb, h->ctr, sizeof(TableCounters), hipMemcpyDeviceToHost, s));
+    QK_HIP(hipStreamSynchronize(s));
+    memcpy(out, hb, sizeof(TableCounters));
+    return QK_SUCCESS;
+}
+
+// Read and clear the latched error (ErrorFlagSwapper::Swap, ErrorFlagSwapper.h:32-48).
+int take_error(qk_groupby_s* h, hipStream_t s) {
+    TableCounters c;
+    QK_TRY(read_counters(h, s, &c));
+    if (c.err != 0) {
+        QK_HIP(hipMemsetAsync(&h->ctr->err, 0, sizeof(int32_t), s));
+        if (c.err == QK_HASH_TABLE_FULL)
+            return fail(QK_HASH_TABLE_FULL, "GROUP BY hash table is full (%llu keys, capacity %llu)",
+                        (unsigned long long)c.nkeys, (unsigned long long)h->logical_cap);
+        return fail(c.err, "GROUP BY kernel error %d", c.err);
+    }
+    return QK_SUCCESS;
+}
+
+int choose_lds_slots(const qk_groupby_s* h) {
+    if (h->ks.kw != 1 || h->lds_budget == 0) return 0;
+    const DeviceInfo& d = device_info();
+    size_t budget = h->lds_budget > 0 ? (size_t)h->lds_budget : d.lds_per_cu / 2;
+    if (budget > d.lds_per_block) budget = d.lds_per_block;
+    const int bps = lds_slot_bytes(h->fields);
+    int64_t slots = (int64_t)(budget / (size_t)bps) - 2;
+    // no point holding more LDS slots than twice the table's key capacity
+    const int64_t useful = (int64_t)std::min<uint64_t>(h->logical_cap * 2 + 64, (uint64_t)1 << 30);
+    if (slots > useful) slots = useful;
+    if (slots < 64) return 0;
+    return (int)slots;
+}
+
+int process_common(qk_groupby h, int mode, const void* const* keys, const uint64_t* const* key_nulls,
+                   const void* values, const uint64_t* value_nulls, const int8_t* mask,
+                   int32_t npreds, const qk_predicate* preds, int32_t n, void* stream) {
+    if (!h) return fail(QK_ERROR, "group by: null table");
+    QK_TRY(ensure_init());
+    if (n <= 0) return QK_SUCCESS;  // GPUGroupBy.cuh:401
+    if (!keys) return fail(QK_ERROR, "group by: null key array");
+    KeyCols kc{};
+    for (int c = 0; c < h->nkeys; c++) {
+        if (!keys[c]) return fail(QK_ERROR, "group by: null key column %d", c);
+        kc.col[c] = keys[c];
+        kc.nm[c] = key_nulls ? key_nulls[c] : nullptr;
+        if (kc.nm[c] && (h->flags & QK_GB_KEYS_NOT_NULL))
+            return fail(QK_ERROR, "group by: key null mask given to a table created with QK_GB_KEYS_NOT_NULL");
+    }
+    bool needs_value = false;
+    for (int a = 0; a < h->naggs; a++) needs_value = needs_value || agg_needs_value(h->aggs[a]);
+    if (needs_value && !values) return fail(QK_ERROR, "group by: aggregate needs a value column");
+    if (mode == 1 && !mask) return fail(QK_ERROR, "group by: null mask");
+    PredSet ps{};
+    QK_TRY(check_preds(npreds, preds, &ps));
+    uint32_t vpred = 0;
+    for (int p = 0; p < ps.n; p++)
+        if (values && ps.p[p].column == values && ps.p[p].type == h->value_type &&
+            ps.p[p].null_mask == value_nulls)
+            vpred |= 1u << p;
+    hipStream_t s = as_stream(stream);
+    const int32_t H = choose_lds_slots(h);
+    h->last_H = H;
+    GView g = view_of(h);
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    if (h->profiling) {
+        if (h->events_used == h->events.size()) {
+            hipEvent_t a, b;
+            QK_HIP(hipEventCreate(&a));
+            QK_HIP(hipEventCreate(&b));
+            h->events.emplace_back(a, b);
+        }
+        ev = &h->events[h->events_used++];
+        QK_HIP(hipEventRecord(ev->first, s));
+    }
+    const int vtype = values ? h->value_type : 0;
+    int st;
+    switch (h->ks.kw) {
+    case 1: st = launch_rows_v<1>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, H, h->block, h->wgs_per_cu, s); break;
+    case 2: st = launch_rows_v<2>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, 0, h->block, h->wgs_per_cu, s); break;
+    default: st = launch_rows_v<4>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, 0, h->block, h->wgs_per_cu, s); break;
+    }
+    if (st != QK_SUCCESS) return st;
+    if (ev) QK_HIP(hipEventRecord(ev->second, s));
+    if (!(h->flags & QK_GB_ASYNC)) return take_error(h, s);
+    return QK_SUCCESS;
+}
+
+int collect_profile(qk_groupby h) {
+    for (size_t i = 0; i < h->events_used; i++) {
+        QK_HIP(hipEventSynchronize(h->events[i].second));
+        float ms = 0.f;
+        QK_HIP(hipEventElapsedTime(&ms, h->events[i].first, h->events[i].second));
+        h->prof_ms += ms;
+        h->prof_launches++;
+    }
+    h->events_used = 0;
+    return QK_SUCCESS;
+}
+
+}  // namespace
+}  // namespace qk
+
+extern "C" {
+
+int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32_t value_type,
+                 int32_t naggs, const int32_t* aggs, const int32_t* out_types,
+                 int32_t max_hash_count, int32_t key_size_multiplier, uint32_t flags) {
+    if (!out) return fail(QK_ERROR, "qk_gb_create: null output");
+    *out = nullptr;
+    QK_TRY(ensure_init());
+    if (nkeys < 1 || nkeys > kMaxKeys) return fail(QK_ERROR, "qk_gb_create: 1..%d key columns", kMaxKeys);
+    if (naggs < 1 || naggs > kMaxAggs) return fail(QK_ERROR, "qk_gb_create: 1..%d aggregates", kMaxAggs);
+    if (!key_types || !aggs) return fail(QK_ERROR, "qk_gb_create: null type/aggregate array");
+    if (max_hash_count < 1 || key_size_multiplier < 1)
+        return fail(QK_ERROR, "qk_gb_create: capacity must be positive");
+    if (flags & QK_GB_AUTO_GROW)
+        return fail(QK_ERROR, "qk_gb_create: QK_GB_AUTO_GROW is not available in this build");
+    std::unique_ptr<qk_groupby_s> h(new qk_groupby_s());
+    h->nkeys = nkeys;
+    h->value_type = value_type;
+    h->naggs = naggs;
+    h->flags = flags;
+    bool needs_value = false;
+    uint32_t fields = F_CNT;
+    for (int a = 0; a < naggs; a++) {
+        const int ag = aggs[a];
+        if (ag < QK_AGG_MIN || ag > QK_AGG_COUNT_ALL)
+            return fail(QK_UNKNOWN_AGG_FUN, "qk_gb_create: unknown aggregate %d", ag);
+        h->aggs[a] = ag;
+        if (agg_needs_value(ag)) needs_value = true;
+        if (ag == QK_AGG_SUM || ag == QK_AGG_AVG) fields |= F_SUM;
+        if (ag == QK_AGG_MIN) fields |= F_MIN;
+        if (ag == QK_AGG_MAX) fields |= F_MAX;
+        if (ag == QK_AGG_COUNT_ALL) fields |= F_CNTALL;
+    }
+    if (needs_value && value_type != QK_INT && value_type != QK_LONG && value_type != QK_FLOAT &&
+        value_type != QK_DOUBLE)
+        return fail(QK_ERROR, "qk_gb_create: value type %d unsupported", value_type);
+    for (int a = 0; a < naggs; a++) {
+        const int ag = aggs[a];
+        int dflt = (ag == QK_AGG_COUNT || ag == QK_AGG_COUNT_ALL) ? QK_LONG : value_type;
+        int ot = (out_types && out_types[a] > 0) ? out_types[a] : dflt;
+        if (ag == QK_AGG_COUNT || ag == QK_AGG_COUNT_ALL) {
+            if (ot != QK_LONG)  // GPUGroupBy.cuh:563-567
+                return fail(QK_ERROR, "Output value data type in GROUP BY with COUNT must be int64_t");
+        } else if (ag == QK_AGG_SUM) {
+            bool widen = (value_type == QK_INT && ot == QK_LONG) || (value_type == QK_FLOAT && ot == QK_DOUBLE);
+            if (ot != value_type && !widen)
+                return fail(QK_ERROR, "Input and output value data type must be the same in GROUP BY");
+        } else if (ag != QK_AGG_NONE && ot != value_type) {  // GPUGroupBy.cuh:524-529
+            return fail(QK_ERROR, "Input and output value data type must be the same in GROUP BY");
+        }
+        h->out_types[a] = ot;
+    }
+    h->fields = fields;
+    h->real_sum = (value_type == QK_FLOAT || value_type == QK_DOUBLE);
+
+    // key layout
+    KeySpec ks{};
+    ks.nkeys = nkeys;
+    int total_bits = 0;
+    for (int c = 0; c < nkeys; c++) {
+        const int t = key_types[c];
+        if (!type_size(t)) return fail(QK_ERROR, "qk_gb_create: key type %d unsupported", t);
+        h->key_types[c] = t;
+        ks.type[c] = t;
+    }
+    if (nkeys == 1) {
+        ks.kw = 1;
+        ks.single = 1;
+    } else {
+        int pos = 0;
+        for (int c = 0; c < nkeys; c++) {
+            const int w = (key_types[c] == QK_LONG || key_types[c] == QK_DOUBLE) ? 64 : 32;
+            if ((pos % 64) + w > 64) pos = (pos / 64 + 1) * 64;
+            ks.word[c] = pos / 64;
+            ks.shift[c] = pos % 64;
+            pos += w;
+        }
+        total_bits = pos;
+        if ((flags & QK_GB_KEYS_NOT_NULL) && total_bits <= 64) {
+            ks.kw = 1;
+        } else {
+            ks.nullable = (flags & QK_GB_KEYS_NOT_NULL) ? 0 : 1;
+            if (ks.nullable)
+                for (int c = 0; c < nkeys; c++) {
+                    ks.nword[c] = pos / 64;
+                    ks.nshift[c] = pos % 64;
+                    pos++;
+                }
+            const int kw = (pos + 1 + 63) / 64;  // + the present bit (bit 63 of the last word)
+            if (ks.nullable) {
+                // the null flags must not collide with the present bit
+                if (pos > kw * 64 - 1) return fail(QK_ERROR, "qk_gb_create: key too wide");
+            }
+            if (kw > 4) return fail(QK_ERROR, "qk_gb_create: composite key wider than 255 bits");
+            ks.kw = kw == 3 ? 4 : kw;
+        }
+    }
+    h->ks = ks;
+
+    // record layout
+    Rec r{};
+    int w = ks.kw;
+    r.kw = ks.kw;
+    r.cnt = w++;
+    r.cntall = (fields & F_CNTALL) ? w++ : -1;
+    r.sum = (fields & F_SUM) ? w++ : -1;
+    r.mn = (fields & F_MIN) ? w++ : -1;
+    r.mx = (fields & F_MAX) ? w++ : -1;
+    r.words = w;
+    h->rec = r;
+
+    const uint64_t gmax = (uint64_t)1 << 30;  // GB_BUFFER_SIZE_MAX (IGroupBy.h:6)
+    uint64_t logical = std::min<uint64_t>((uint64_t)max_hash_count * (uint64_t)key_size_multiplier, gmax);
+    h->logical_cap = logical;
+    h->reported_max = (int32_t)(nkeys == 1 ? logical + 1 : logical);  // + 1 for the NULL key
+    uint64_t cap = 64;
+    while (cap < 2 * logical) cap <<= 1;
+    h->cap = cap;
+    QK_TRY(pool_alloc((void**)&h->slots, (size_t)(cap + 2) * r.words * 8));
+    int st = pool_alloc((void**)&h->ctr, sizeof(TableCounters));
+    if (st != QK_SUCCESS) {
+        pool_free(h->slots);
+        return st;
+    }
+    QK_HIP(hipGetDevice(&h->device));
+    st = init_table(h.get(), nullptr);
+    if (st == QK_SUCCESS) {
+        hipError_t e = hipStreamSynchronize(nullptr);
+        if (e != hipSuccess) st = fail(QK_RUNTIME_ERROR, "table init: %s", hipGetErrorString(e));
+    }
+    if (st != QK_SUCCESS) {
+        pool_free(h->slots);
+        pool_free(h->ctr);
+        return st;
+    }
+    *out = h.release();
+    return QK_SUCCESS;
+}
+
+int qk_gb_destroy(qk_groupby h) {
+    if (!h) return QK_SUCCESS;
+    (void)hipDeviceSynchronize();
+    pool_free(h->slots);
+    pool_free(h->ctr);
+    for (auto& e : h->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    delete h;
+    return QK_SUCCESS;
+}
+
+int qk_gb_reset(qk_groupby h, void* stream) {
+    if (!h) return fail(QK_ERROR, "qk_gb_reset: null table");
+    return init_table(h, as_stream(stream));
+}
+
+int qk_gb_get_max_hash_count(qk_groupby h, int32_t* out) {
+    if (!h || !out) return fail(QK_ERROR, "qk_gb_get_max_hash_count: null argument");
+    *out = h->reported_max;
+    return QK_SUCCESS;
+}
+
+int qk_gb_process_block(qk_groupby h, const void* const* keys, const uint64_t* const* key_nulls,
+                        const void* values, const uint64_t* value_nulls, int32_t n, void* stream) {
+    return process_common(h, 0, keys, key_nulls, values, value_nulls, nullptr, 0, nullptr, n, stream);
+}
+
+int qk_gb_process_block_masked(qk_groupby h, const void* const* keys,
+                               const uint64_t* const* key_nulls, const void* values,
+                               const uint64_t* value_nulls, const int8_t* mask, int32_t n,
+                               void* stream) {
+    return process_common(h, 1, keys, key_nulls, values, value_nulls, mask, 0, nullptr, n, stream);
+}
+
+int qk_gb_process_block_where(qk_groupby h, const void* const* keys,
+                              const uint64_t* const* key_nulls, const void* values,
+                              const uint64_t* value_nulls, int32_t n, int32_t npreds,
+                              const qk_predicate* preds, void* stream) {
+    return process_common(h, npreds > 0 ? 2 : 0, keys, key_nulls, values, value_nulls, nullptr,
+                          npreds, preds, n, stream);
+}
+
+int qk_gb_check_error(qk_groupby h, void* stream) {
+    if (!h) return fail(QK_ERROR, "qk_gb_check_error: null table");
+    return take_error(h, as_stream(stream));
+}
+
+int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream) {
+    if (!h || !out) return fail(QK_ERROR, "qk_gb_group_count: null argument");
+    hipStream_t s = as_stream(stream);
+    TableCounters c;
+    QK_TRY(read_counters(h, s, &c));
+    uint64_t side = 0;
+    if (h->ks.kw == 1) {
+        uint64_t w[1];
+        void* hb = nullptr;
+        QK_TRY(pinned_host(8, &hb));
+        QK_HIP(hipMemcpyAsync(hb, h->slots + h->cap * h->rec.words, 8, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+        memcpy(w, hb, 8);
+        side = w[0] != 0 ? 1 : 0;  // the NULL group
+    }
+    *out = (int64_t)(c.nkeys + side);
+    return QK_SUCCESS;
+}
+
+int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, void** out_values,
+                      uint64_t** out_value_nulls, int32_t* out_count, void* stream) {
+    if (!h || !out_count) return fail(QK_ERROR, "qk_gb_get_results: null argument");
+    if (!out_keys) return fail(QK_ERROR, "qk_gb_get_results: null key output array");
+    QK_TRY(ensure_init());
+    hipStream_t s = as_stream(stream);
+    *out_count = 0;
+    for (int c = 0; c < h->nkeys; c++) {
+        out_keys[c] = nullptr;
+        if (out_key_nulls) out_key_nulls[c] = nullptr;
+    }
+    for (int a = 0; a < h->naggs; a++) {
+        if (out_values) out_values[a] = nullptr;
+        if (out_value_nulls) out_value_nulls[a] = nullptr;
+    }
+    QK_TRY(take_error(h, s));
+    int64_t groups = 0;
+    QK_TRY(qk_gb_group_count(h, &groups, stream));
+    if (groups == 0) return QK_SUCCESS;
+    if (groups > INT32_MAX) return fail(QK_ERROR, "qk_gb_get_results: too many groups");
+
+    OutSpec os{};
+    os.naggs = h->naggs;
+    os.vtype = h->value_type;
+    std::vector<void*> owned;
+    auto cleanup = [&]() {
+        for (void* p : owned) pool_free(p);
+    };
+    auto alloc = [&](size_t bytes, void** p) -> int {
+        int st = pool_alloc(p, bytes);
+        if (st == QK_SUCCESS) owned.push_back(*p);
+        return st;
+    };
+    const size_t nwords = (size_t)((groups + 63) / 64);
+    int st = QK_SUCCESS;
+    for (int c = 0; c < h->nkeys && st == QK_SUCCESS; c++) {
+        st = alloc((size_t)groups * type_size(h->key_types[c]), &os.kout[c]);
+        if (st == QK_SUCCESS && out_key_nulls) {
+            st = alloc(nwords * 8, (void**)&os.konull[c]);
+            if (st == QK_SUCCESS) st = hipMemsetAsync(os.konull[c], 0, nwords * 8, s) == hipSuccess ? QK_SUCCESS : QK_RUNTIME_ERROR;
+        }
+    }
+    for (int a = 0; a < h->naggs && st == QK_SUCCESS; a++) {
+        os.agg[a] = h->aggs[a];
+        os.otype[a] = h->out_types[a];
+        if (h->aggs[a] == QK_AGG_NONE || !out_values) continue;
+        st = alloc((size_t)groups * type_size(h->out_types[a]), &os.out[a]);
+        if (st == QK_SUCCESS && out_value_nulls) {
+            st = alloc(nwords * 8, (void**)&os.onull[a]);
+            if (st == QK_SUCCESS) st = hipMemsetAsync(os.onull[a], 0, nwords * 8, s) == hipSuccess ? QK_SUCCESS : QK_RUNTIME_ERROR;
+        }
+    }
+    if (st != QK_SUCCESS) {
+        cleanup();
+        return st == QK_RUNTIME_ERROR ? fail(st, "qk_gb_get_results: memset failed") : st;
+    }
+    const int64_t nslots = (int64_t)h->cap + 2;
+    const int64_t tiles = (nslots + kEmitTile - 1) / kEmitTile;
+    Scratch* sc = nullptr;
+    st = stream_scratch(s, lb_scratch_bytes(tiles) + 64, &sc);
+    if (st != QK_SUCCESS) {
+        cleanup();
+        return st;
+    }
+    uint64_t* words = (uint64_t*)sc->ptr;
+    int32_t* total = (int32_t*)(words + kLbHeaderWords + tiles);
+    const uint32_t epoch = ++sc->epoch;
+    GView g = view_of(h);
+    k_gb_emit<<<(unsigned)tiles, kEmitBlock, 0, s>>>(g, h->ks, os, words + kLbHeaderWords,
+                                                     (uint32_t*)words, epoch, tiles, total,
+                                                     groups);
+    hipError_t e = hipGetLastError();
+    int32_t written = 0;
+    if (e == hipSuccess) {
+        void* hb = nullptr;
+        st = pinned_host(8, &hb);
+        if (st == QK_SUCCESS) {
+            e = hipMemcpyAsync(hb, total, 4, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess) written = *(int32_t*)hb;
+        }
+    }
+    if (e != hipSuccess || st != QK_SUCCESS) {
+        cleanup();
+        return st != QK_SUCCESS ? st : fail(QK_RUNTIME_ERROR, "qk_gb_get_results: %s", hipGetErrorString(e));
+    }
+    if (written != (int32_t)groups) {
+        cleanup();
+        return fail(QK_ERROR, "qk_gb_get_results: %d groups emitted, %lld counted", written, (long long)groups);
+    }
+    for (int c = 0; c < h->nkeys; c++) {
+        out_keys[c] = os.kout[c];
+        if (out_key_nulls) out_key_nulls[c] = os.konull[c];
+    }
+    for (int a = 0; a < h->naggs; a++) {
+        if (out_values) out_values[a] = os.out[a];
+        if (out_value_nulls) out_value_nulls[a] = os.onull[a];
+    }
+    *out_count = (int32_t)groups;
+    return QK_SUCCESS;
+}
+
+int qk_gb_set_profiling(qk_groupby h, int32_t enable) {
+    if (!h) return fail(QK_ERROR, "qk_gb_set_profiling: null table");
+    h->profiling = enable != 0;
+    return QK_SUCCESS;
+}
+
+int qk_gb_get_profile(qk_groupby h, double* ms, int64_t* launches) {
+    if (!h) return fail(QK_ERROR, "qk_gb_get_profile: null table");
+    QK_TRY(collect_profile(h));
+    if (ms) *ms = h->prof_ms;
+    if (launches) *launches = h->prof_launches;
+    h->prof_ms = 0.0;
+    h->prof_launches = 0;
+    return QK_SUCCESS;
+}
+
+int qk_gb_describe(qk_groupby h, int32_t* slot_bytes, int64_t* slots, int32_t* lds_slots) {
+    if (!h) return fail(QK_ERROR, "qk_gb_describe: null table");
+    if (slot_bytes) *slot_bytes = h->rec.words * 8;
+    if (slots) *slots = (int64_t)h->cap;
+    if (lds_slots) *lds_slots = h->last_H;
+    return QK_SUCCESS;
+}
+
+int qk_gb_set_tuning(qk_groupby h, int32_t block, int32_t lds_bytes, int32_t wgs_per_cu) {
+    if (!h) return fail(QK_ERROR, "qk_gb_set_tuning: null table");
+    if (block > 0) {
+        if (block % 64 != 0 || block > 1024) return fail(QK_ERROR, "qk_gb_set_tuning: block must be a multiple of 64 up to 1024");
+        h->block = block;
+    }
+    if (lds_bytes >= 0) h->lds_budget = lds_bytes;
+    if (wgs_per_cu >= 0) h->wgs_per_cu = wgs_per_cu;
+    return QK_SUCCESS;
+}
+
+}  // extern "C"

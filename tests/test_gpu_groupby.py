@@ -1,0 +1,255 @@
+"""GPU parity of GROUP BY against the oracle: the reference's known answers, random inputs over
+every key/value type and aggregate, NULL handling, composite keys, table-full behaviour,
+and the benchmark workloads (bit-exact integers; FP32 SUM/AVG within 1e-5 relative of FP64)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import qikkdb_amd as qk
+from qikkdb_amd import datagen
+from gpu_util import Where, cpu_group_by, gpu_group_by
+from helpers import AGG_NAMES, as_map, expect_map, golden, maps_close
+
+pytestmark = pytest.mark.gpu
+G = golden()
+FP_REL = 1e-5  # FP32 SUM/AVG tolerance against the FP64 oracle
+
+
+def compare(gpu, cpu, naggs, real_aggs=()):
+    for a in range(naggs):
+        maps_close(as_map(gpu, a), as_map(cpu, a), rel=FP_REL if a in real_aggs else 0.0)
+
+
+# ---- reference known answers ----------------------------------------------------------------
+
+@pytest.mark.parametrize("case", G["group_by_single"], ids=lambda c: c["source"].split()[1])
+def test_known_answers_single(case):
+    keys = np.array(case["keys"], np.int32)
+    vals = np.array(case["values"], np.int32)
+    kn = [np.array(case["key_nulls"])] if "key_nulls" in case else None
+    vn = np.array(case["value_nulls"]) if "value_nulls" in case else None
+    where, extra = (), None
+    if "where" in case:
+        w = case["where"]
+        if w["column"] == "keys":
+            where = [Where(getattr(qk, w["op"]), "key0", w["const"], qk.INT)]
+        else:
+            extra = {"filter": (np.array(case["filter"], np.int32), None)}
+            where = [Where(getattr(qk, w["op"]), "filter", w["const"], qk.INT)]
+    agg = AGG_NAMES[case["agg"]]
+    res = gpu_group_by(agg, [keys], None if case.get("count_star") else vals, kn,
+                       None if case.get("count_star") else vn, where=where, block=case["block"],
+                       capacity=262144, extra_cols=extra, value_type=qk.INT)
+    maps_close(as_map(res), expect_map(case["expect"], False))
+
+
+@pytest.mark.parametrize("case", G["group_by_multi"], ids=lambda c: c["source"].split()[1])
+def test_known_answers_multi(case):
+    a = np.array(case["a"], np.int32)
+    b = np.array(case["b"], np.int32)
+    vals = np.array(case["values"], np.int32)
+    kn = [np.array(case["a_nulls"]), np.array(case["b_nulls"])] if "a_nulls" in case else None
+    vn = np.array(case["value_nulls"]) if "value_nulls" in case else None
+    res = gpu_group_by(AGG_NAMES[case["agg"]], [a, b], None if case.get("count_star") else vals, kn,
+                       None if case.get("count_star") else vn, block=case["block"],
+                       capacity=case.get("max_hash_count", 262144), value_type=qk.INT)
+    maps_close(as_map(res), expect_map(case["expect"], True))
+
+
+def test_whole_column_float_sum_known_answer():
+    case = G["whole_column"][0]
+    v = np.full(case["n"], case["value"], np.float32)
+    res = gpu_group_by(qk.AGG_SUM, [np.zeros(case["n"], np.int32)], v)
+    assert as_map(res) == {(0,): case["sum"]}
+
+
+# ---- random parity over types ----------------------------------------------------------------
+
+ALL_AGGS = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_AVG, qk.AGG_COUNT, qk.AGG_COUNT_ALL]
+
+
+def make_keys(dt, n, card, rng):
+    if np.issubdtype(dt, np.integer):
+        base = rng.integers(0, card, n)
+        return (base * 7919 - card * 3000).astype(dt)
+    return (rng.integers(0, card, n) * 0.25 - card / 8).astype(dt)
+
+
+def make_vals(dt, n, rng):
+    if np.issubdtype(dt, np.integer):
+        return rng.integers(-10**6, 10**6, n).astype(dt)
+    return rng.lognormal(0, 1, n).astype(dt)
+
+
+@pytest.mark.parametrize("kt", [np.int32, np.int64, np.float32, np.float64])
+@pytest.mark.parametrize("vt", [np.int32, np.int64, np.float32, np.float64])
+@pytest.mark.parametrize("card", [7, 3000, 200_000])
+def test_random_types(kt, vt, card):
+    rng = np.random.default_rng(card + 17 * np.dtype(kt).itemsize + np.dtype(vt).itemsize)
+    n = 400_000
+    keys = make_keys(kt, n, card, rng)
+    vals = make_vals(vt, n, rng)
+    kn = [rng.random(n) < 0.02]
+    vn = rng.random(n) < 0.1
+    aggs = [qk.AGG_COUNT, qk.AGG_SUM, qk.AGG_AVG, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT_ALL]
+    real = (1, 2) if np.issubdtype(vt, np.floating) else ()
+    gpu = gpu_group_by(aggs, [keys], vals, kn, vn, block=1 << 17, capacity=card * 2 + 16)
+    cpu = cpu_group_by(aggs, [keys], vals, kn, vn)
+    compare(gpu, cpu, len(aggs), real)
+
+
+@pytest.mark.parametrize("lds", [0, 4096, -1])
+def test_lds_stage_does_not_change_results(lds):
+    rng = np.random.default_rng(lds + 5)
+    n = 1_000_000
+    keys = datagen.zipf_keys(11, 1.1, 50_000, n)
+    vals = rng.integers(-1000, 1000, n).astype(np.int64)
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    tuning = None if lds < 0 else {"lds_bytes": lds}
+    gpu = gpu_group_by(aggs, [keys], vals, capacity=100_000, tuning=tuning)
+    cpu = cpu_group_by(aggs, [keys], vals)
+    compare(gpu, cpu, len(aggs))
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_block_shapes(block):
+    rng = np.random.default_rng(block)
+    n = 300_001
+    keys = rng.integers(0, 5000, n).astype(np.int32)
+    vals = rng.integers(-1000, 1000, n).astype(np.int32)
+    aggs = [qk.AGG_SUM, qk.AGG_COUNT]
+    gpu = gpu_group_by(aggs, [keys], vals, capacity=10_000, tuning={"block": block})
+    compare(gpu, cpu_group_by(aggs, [keys], vals), 2)
+
+
+def test_masked_and_where_forms_agree_with_oracle():
+    rng = np.random.default_rng(21)
+    n = 500_000
+    keys = rng.integers(0, 1000, n).astype(np.int32)
+    vals = rng.integers(-1000, 1001, n).astype(np.int32)
+    other = rng.integers(-1000, 1001, n).astype(np.int32)
+    mask = (vals > 0).astype(np.int8)
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    cpu = cpu_group_by(aggs, [keys], vals, mask=mask)
+    compare(gpu_group_by(aggs, [keys], vals, mask=mask, capacity=2000), cpu, 4)
+    w = [Where(qk.GREATER, "value", 0, qk.INT)]
+    compare(gpu_group_by(aggs, [keys], vals, where=w, capacity=2000), cpu, 4)
+    # column-vs-column predicate on a column that is not the value
+    w2 = [Where(qk.LESS, "other", None, None, rhs="value")]
+    extra = {"other": (other, None)}
+    compare(gpu_group_by(aggs, [keys], vals, where=w2, extra_cols=extra, capacity=2000),
+            cpu_group_by(aggs, [keys], vals, where=w2, extra_cols=extra), 4)
+
+
+def test_edge_keys_sentinel_and_signed_zero():
+    keys = np.array([np.iinfo(np.int64).min] * 3 + [0, -1, np.iinfo(np.int64).max], np.int64)
+    vals = np.arange(6, dtype=np.int64)
+    gpu = gpu_group_by([qk.AGG_SUM, qk.AGG_COUNT], [keys], vals, capacity=4)
+    assert as_map(gpu, 0) == {(np.iinfo(np.int64).min,): 3, (0,): 3, (-1,): 4, (np.iinfo(np.int64).max,): 5}
+    fk = np.array([0.0, -0.0, 1.5, -1.5], np.float32)
+    gpu = gpu_group_by(qk.AGG_COUNT, [fk], fk, capacity=4)
+    assert as_map(gpu) == {(0.0,): 2, (1.5,): 1, (-1.5,): 1}
+
+
+def test_table_full_is_reported_and_capacity_independent():
+    rng = np.random.default_rng(4)
+    n = 100_000
+    keys = rng.integers(0, 5000, n).astype(np.int32)
+    vals = rng.integers(0, 10, n).astype(np.int32)
+    with pytest.raises(qk.QueryEngineError) as e:
+        gpu_group_by(qk.AGG_SUM, [keys], vals, capacity=1000)
+    assert e.value.code == qk.QK_HASH_TABLE_FULL
+    ref = as_map(cpu_group_by(qk.AGG_SUM, [keys], vals))
+    for cap in (5000, 6000, 1 << 20):
+        maps_close(as_map(gpu_group_by(qk.AGG_SUM, [keys], vals, capacity=cap)), ref)
+
+
+def test_max_hash_count_and_empty_results():
+    gb = qk.GPUGroupBy(qk.AGG_SUM, qk.INT, qk.INT, 262144, 2)
+    assert gb.GetMaxHashCount() == 262144 * 2 + 1
+    r = gb.GetResults()
+    assert r.count == 0 and r.keys[0] is None and r.values[0] is None
+    gb.close()
+    gb = qk.GPUGroupBy(qk.AGG_SUM, [qk.INT, qk.INT], qk.INT, 8, 1)
+    assert gb.GetMaxHashCount() == 8
+    gb.close()
+
+
+def test_type_rules_rejected_like_reference():
+    with pytest.raises(qk.QueryEngineError):
+        qk.GPUGroupBy(qk.AGG_COUNT, qk.INT, qk.INT, 16, 1, out_types=[qk.INT])
+    with pytest.raises(qk.QueryEngineError):
+        qk.GPUGroupBy(qk.AGG_MIN, qk.INT, qk.INT, 16, 1, out_types=[qk.LONG])
+
+
+# ---- composite keys ----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("flags", [0, qk.GB_KEYS_NOT_NULL])
+def test_composite_int_int(flags):
+    rng = np.random.default_rng(flags + 3)
+    n = 600_000
+    a = rng.integers(0, 300, n).astype(np.int32)
+    b = rng.integers(-300, 300, n).astype(np.int32)
+    vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    kn = None if flags else [rng.random(n) < 0.05, rng.random(n) < 0.05]
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT, qk.AGG_AVG]
+    gpu = gpu_group_by(aggs, [a, b], vals, kn, capacity=200_000, flags=flags)
+    cpu = cpu_group_by(aggs, [a, b], vals, kn)
+    compare(gpu, cpu, len(aggs))
+
+
+def test_composite_wide_keys():
+    rng = np.random.default_rng(8)
+    n = 300_000
+    k1 = (rng.integers(0, 50, n) * (1 << 40)).astype(np.int64)
+    k2 = rng.integers(0, 50, n).astype(np.float64)
+    k3 = rng.integers(-5, 5, n).astype(np.int32)
+    vals = rng.lognormal(0, 1, n).astype(np.float64)
+    kn = [rng.random(n) < 0.02, None, rng.random(n) < 0.02]
+    aggs = [qk.AGG_SUM, qk.AGG_COUNT, qk.AGG_MAX]
+    gpu = gpu_group_by(aggs, [k1, k2, k3], vals, kn, capacity=50_000)
+    cpu = cpu_group_by(aggs, [k1, k2, k3], vals, kn)
+    compare(gpu, cpu, len(aggs), real_aggs=(0,))
+
+
+# ---- benchmark workloads -----------------------------------------------------------------------
+
+def test_workload_small_full_size_bit_exact():
+    d = datagen.small(1 << 20, seed=1)
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    w = [Where(qk.GREATER, "value", 0, qk.INT)]
+    out_types = [qk.LONG, 0, 0, 0]
+    gpu = gpu_group_by(aggs, [d["key"]], d["value"], where=w, capacity=1000, out_types=out_types)
+    cpu = cpu_group_by(aggs, [d["key"]], d["value"], where=w, out_types=out_types)
+    assert gpu["count"] == 1000
+    compare(gpu, cpu, 4)
+
+
+def _unpack(words, n):
+    return np.unpackbits(words.view(np.uint8), bitorder="little")[:n].astype(bool)
+
+
+@pytest.mark.parametrize("n", [1 << 22])
+def test_workload_zipf_fp_tolerance(n):
+    d = datagen.zipf(n, seed=2)
+    vn = _unpack(d["value_nulls"], n)
+    aggs = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT]
+    w = [Where(qk.GREATER_EQUAL, "value", 0.5, qk.FLOAT), Where(qk.LESS_EQUAL, "value", 4.0, qk.FLOAT)]
+    gpu = gpu_group_by(aggs, [d["key"]], d["value"], None, vn, where=w, block=1 << 20,
+                       capacity=1_000_000, flags=qk.GB_KEYS_NOT_NULL)
+    cpu = cpu_group_by(aggs, [d["key"]], d["value"], None, vn, where=w)
+    compare(gpu, cpu, 3, real_aggs=(0, 1))
+
+
+def test_workload_composite_bit_exact():
+    n = 1 << 23
+    d = datagen.composite(n, seed=3)
+    ts = d["ts"]
+    w = [Where(qk.GREATER_EQUAL, "ts", datagen.WINDOW_START_MS, qk.LONG),
+         Where(qk.LESS, "ts", datagen.WINDOW_END_MS, qk.LONG)]
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    extra = {"ts": (ts, None)}
+    gpu = gpu_group_by(aggs, [d["a"], d["b"]], d["value"], where=w, block=1 << 22, capacity=1 << 24,
+                       flags=qk.GB_KEYS_NOT_NULL, extra_cols=extra)
+    cpu = cpu_group_by(aggs, [d["a"], d["b"]], d["value"], where=w, extra_cols=extra)
+    compare(gpu, cpu, 4)
