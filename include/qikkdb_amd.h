@@ -278,6 +278,12 @@ int qk_gb_describe(qk_groupby h, int32_t* slot_bytes, int64_t* slots, int32_t* l
  * that setting unchanged. */
 int qk_gb_set_tuning(qk_groupby h, int32_t block, int32_t lds_bytes, int32_t wgs_per_cu);
 
+/* Row-kernel path for single-word keys (no reference counterpart): path -1 = automatic
+ * (two-phase partitioned aggregation for blocks of at least min_rows rows, the one-pass
+ * kernel below that), 0 = always one-pass, 1 = always two-phase.  min_rows < 0 leaves the
+ * threshold unchanged.  Results do not depend on the path. */
+int qk_gb_set_path(qk_groupby h, int32_t path, int64_t min_rows);
+
 #ifdef __cplusplus
 }
 #endif

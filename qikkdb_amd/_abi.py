@@ -110,6 +110,7 @@ PROTOTYPES = {
     "qk_gb_get_profile": [_vp, _P(C.c_double), _P(_i64)],
     "qk_gb_describe": [_vp, _P(_i32), _P(_i64), _P(_i32)],
     "qk_gb_set_tuning": [_vp, _i32, _i32, _i32],
+    "qk_gb_set_path": [_vp, _i32, _i64],
 }
 
 _lib = None

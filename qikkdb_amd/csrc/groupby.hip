@@ -3,16 +3,18 @@
 // Counterpart of GPUGroupBy<AGG,O,K,V> (QueryEngine/GPUCore/GPUGroupBy.cuh:68-791) and its
 // composite-key specialisation (GPUGroupByMultiKey.cuh:156-1325).  Design differences:
 //  * One table keeps several aggregates per group (the reference keeps one, S11).
-//  * Every workgroup first aggregates into its own open-addressing table in LDS (bounded
-//    linear probing, native LDS atomics); a row whose key finds no LDS slot goes straight to
-//    the global table, and the LDS table is merged into the global one when the workgroup
-//    ends.  The reference instead spreads each slot over 256 replicas in HBM
-//    (IGroupBy.h:5, GPUGroupBy.cuh:181-191) and folds them in GetResults.
-//  * Global slots are records (key words + counters + accumulators) so one probe touches one
-//    line; single-word keys are claimed with a 64-bit CAS, multi-word keys by claiming the
-//    last word (empty -> busy -> key).
-//  * MIN/MAX of every type use native unsigned 64-bit atomics on an order-preserving
-//    encoding; SUM of FLOAT/DOUBLE accumulates in FP64, integer SUM in wrapping int64.
+//  * Single-word keys (one key column, or composite keys packed into 64 bits) go through the
+//    two-phase partitioned path (twophase.hip): heavy hitters aggregate in LDS, every other
+//    row is partitioned without atomics and merged by the partition's owning workgroup.  The
+//    reference instead spreads each slot over 256 replicas in HBM (IGroupBy.h:5,
+//    GPUGroupBy.cuh:181-191) and folds them in GetResults.
+//  * Multi-word keys and small blocks use a one-pass kernel: a per-workgroup LDS table in
+//    front of the global table, merged with atomics at the end.
+//  * The global table is a structure of arrays: key words, counts, sums, minima, maxima.
+//    Single-word keys are claimed with a 64-bit CAS; multi-word keys by claiming the last word
+//    (empty -> busy -> key).  MIN/MAX of every type use native unsigned 64-bit atomics on an
+//    order-preserving encoding; SUM of FLOAT/DOUBLE accumulates in FP64, integer SUM in
+//    wrapping int64.
 //  * Results come out in slot order through a look-back compaction, in one kernel.
 #include <algorithm>
 #include <cstring>
@@ -22,10 +24,9 @@
 
 #include "groupby.hpp"
 #include "lookback.hpp"
+#include "twophase.hpp"
 
 namespace qk {
-
-struct NoValue {};
 
 namespace {
 
@@ -35,197 +36,7 @@ constexpr int kEmitBlock = 256;
 constexpr int kEmitSteps = 16;
 constexpr int kEmitTile = kEmitBlock * kEmitSteps;
 
-// ---- key normalisation -----------------------------------------------------------------
-
-// Bits of a FLOAT/DOUBLE key with -0.0 folded into +0.0 (so equal values group together).
-__device__ __forceinline__ uint64_t fkey32(float f) {
-    uint32_t b = __float_as_uint(f);
-    return (b & 0x7fffffffu) ? (uint64_t)b : 0ull;
-}
-__device__ __forceinline__ uint64_t fkey64(double d) {
-    uint64_t b = (uint64_t)__double_as_longlong(d);
-    return (b & 0x7fffffffffffffffull) ? b : 0ull;
-}
-
-// Single key column widened to one 64-bit word.
-__device__ __forceinline__ uint64_t key_single(int type, const void* col, int64_t r) {
-    switch (type) {
-    case QK_INT: return (uint64_t)(int64_t)((const int32_t*)col)[r];
-    case QK_LONG: return (uint64_t)((const int64_t*)col)[r];
-    case QK_FLOAT: return fkey32(((const float*)col)[r]);
-    case QK_DOUBLE: return fkey64(((const double*)col)[r]);
-    default: return (uint64_t)(int64_t)((const int8_t*)col)[r];
-    }
-}
-
-// Field bits of a composite key column: 32-bit fields for INT/FLOAT/INT8, 64-bit otherwise.
-__device__ __forceinline__ uint64_t key_field(int type, const void* col, int64_t r) {
-    switch (type) {
-    case QK_INT: return (uint64_t)(uint32_t)((const int32_t*)col)[r];
-    case QK_LONG: return (uint64_t)((const int64_t*)col)[r];
-    case QK_FLOAT: return fkey32(((const float*)col)[r]);
-    case QK_DOUBLE: return fkey64(((const double*)col)[r]);
-    default: return (uint64_t)(uint32_t)(int32_t)((const int8_t*)col)[r];
-    }
-}
-
-template <int KW>
-__device__ __forceinline__ void load_key(const KeySpec& ks, const KeyCols& kc, int64_t r,
-                                         uint64_t (&k)[KW], bool& knull) {
-    knull = false;
-    if (KW == 1 && ks.single) {
-        knull = kc.nm[0] && null_bit(kc.nm[0], r);
-        k[0] = knull ? 0ull : key_single(ks.type[0], kc.col[0], r);
-        return;
-    }
-#pragma unroll
-    for (int w = 0; w < KW; w++) k[w] = 0;
-    for (int c = 0; c < ks.nkeys; c++) {
-        const bool isnull = kc.nm[c] && null_bit(kc.nm[c], r);
-        if (isnull) {
-            if (KW > 1) {
-#pragma unroll
-                for (int w = 0; w < KW; w++)
-                    if (w == ks.nword[c]) k[w] |= 1ull << ks.nshift[c];
-            }
-        } else {
-            const uint64_t f = key_field(ks.type[c], kc.col[c], r) << ks.shift[c];
-#pragma unroll
-            for (int w = 0; w < KW; w++)
-                if (w == ks.word[c]) k[w] |= f;
-        }
-    }
-    if (KW > 1) k[KW - 1] |= kPresent;
-}
-
-template <int KW>
-__device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[KW]) {
-    uint64_t h = hash64(k[0]);
-#pragma unroll
-    for (int w = 1; w < KW; w++) h = hash64(h ^ (k[w] + 0x9e3779b97f4a7c15ull * (uint64_t)w));
-    return h;
-}
-
-// ---- value handling ----------------------------------------------------------------------
-
-template <typename VT> struct VTraits {
-    static constexpr bool has = true;
-    static constexpr bool real = std::is_floating_point<VT>::value;
-};
-template <> struct VTraits<NoValue> {
-    static constexpr bool has = false;
-    static constexpr bool real = false;
-};
-
-// Partial aggregate of one row or one LDS slot.
-struct Part {
-    uint64_t cnt, cntall, sum, mn, mx;  // sum holds FP64 bits or wrapping int64
-};
-
-// ---- global table --------------------------------------------------------------------
-
-__device__ __forceinline__ void g_new_key(const GView& g) {
-    unsigned long long c = atomicAdd(g.nkeys, 1ull) + 1ull;
-    if (c > g.logical_cap)
-        __hip_atomic_store(g.err, (int32_t)QK_HASH_TABLE_FULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void g_claim_side(const GView& g, uint64_t s, bool counts) {
-    uint64_t* kp = rec_ptr(g, s);
-    if (*(volatile uint64_t*)kp == 0ull) {
-        unsigned long long old = atomicCAS((unsigned long long*)kp, 0ull, 1ull);
-        if (old == 0ull && counts) g_new_key(g);
-    }
-}
-
-// Slot of a single-word key (inserting it when absent); -1 when the table has no free slot.
-__device__ __forceinline__ int64_t g_slot1(const GView& g, uint64_t key, bool knull, uint64_t h) {
-    if (knull) {
-        g_claim_side(g, g.cap, false);
-        return (int64_t)g.cap;
-    }
-    if (key == kEmpty) {
-        g_claim_side(g, g.cap + 1, true);
-        return (int64_t)(g.cap + 1);
-    }
-    const uint64_t m = g.cap - 1;
-    uint64_t s = h & m;
-    for (uint64_t p = 0; p < g.cap; p++) {
-        uint64_t* kp = rec_ptr(g, s);
-        uint64_t cur = *(volatile uint64_t*)kp;  // a stale kEmpty only costs a CAS
-        if (cur == key) return (int64_t)s;
-        if (cur == kEmpty) {
-            unsigned long long old = atomicCAS((unsigned long long*)kp, (unsigned long long)kEmpty,
-                                               (unsigned long long)key);
-            if (old == kEmpty) {
-                g_new_key(g);
-                return (int64_t)s;
-            }
-            if (old == key) return (int64_t)s;
-        }
-        s = (s + 1) & m;
-    }
-    return -1;
-}
-
-// Slot of a multi-word key: the last word is claimed (0 -> kBusy), the other words written,
-// then the last word published with release order; readers that see kBusy wait for it.
-template <int KW>
-__device__ __forceinline__ int64_t g_slotN(const GView& g, const uint64_t (&k)[KW], uint64_t h) {
-    const uint64_t m = g.cap - 1;
-    uint64_t s = h & m;
-    for (uint64_t p = 0; p < g.cap; p++) {
-        uint64_t* kp = rec_ptr(g, s);
-        uint64_t last = __hip_atomic_load(kp + (KW - 1), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (last == 0ull) {
-            unsigned long long old = atomicCAS((unsigned long long*)(kp + (KW - 1)), 0ull,
-                                               (unsigned long long)kBusy);
-            if (old == 0ull) {
-#pragma unroll
-                for (int w = 0; w < KW - 1; w++) st_relaxed(kp + w, k[w]);
-                __hip_atomic_store(kp + (KW - 1), k[KW - 1], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                g_new_key(g);
-                return (int64_t)s;
-            }
-            last = old;
-        }
-        while (last == kBusy) {
-            __builtin_amdgcn_s_sleep(1);
-            last = __hip_atomic_load(kp + (KW - 1), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (last == k[KW - 1]) {
-            bool eq = true;
-#pragma unroll
-            for (int w = 0; w < KW - 1; w++) eq = eq && (ld_relaxed(kp + w) == k[w]);
-            if (eq) return (int64_t)s;
-        }
-        s = (s + 1) & m;
-    }
-    return -1;
-}
-
-__device__ __forceinline__ void g_accum(const GView& g, int64_t s, const Part& p, uint32_t fields) {
-    uint64_t* r = rec_ptr(g, (uint64_t)s);
-    if (p.cnt) atomicAdd((unsigned long long*)(r + g.rec.cnt), (unsigned long long)p.cnt);
-    if ((fields & F_CNTALL) && p.cntall)
-        atomicAdd((unsigned long long*)(r + g.rec.cntall), (unsigned long long)p.cntall);
-    if (p.cnt) {
-        if (fields & F_SUM) {
-            if (g.real_sum)
-                atomicAdd((double*)(r + g.rec.sum), __longlong_as_double((long long)p.sum));
-            else
-                atomicAdd((unsigned long long*)(r + g.rec.sum), (unsigned long long)p.sum);
-        }
-        if (fields & F_MIN) atomicMin((unsigned long long*)(r + g.rec.mn), (unsigned long long)p.mn);
-        if (fields & F_MAX) atomicMax((unsigned long long*)(r + g.rec.mx), (unsigned long long)p.mx);
-    }
-}
-
-__device__ __forceinline__ void g_fail(const GView& g) {
-    __hip_atomic_store(g.err, (int32_t)QK_HASH_TABLE_FULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---- LDS table ---------------------------------------------------------------------------
+// ---- LDS table of the one-pass kernel ----------------------------------------------------
 // Arrays of H hashed slots + 2 side slots (NULL key, key == kEmpty).
 struct Lds {
     uint64_t* key;
@@ -265,7 +76,6 @@ __device__ __forceinline__ int lds_find(const Lds& t, uint64_t key, bool knull, 
         if (t.key[t.H + 1] == 0ull) t.key[t.H + 1] = 1ull;
         return t.H + 1;
     }
-    // range reduction of the high hash bits onto [0, H)
     uint32_t s = (uint32_t)(((h >> 32) * (uint64_t)(uint32_t)t.H) >> 32);
 #pragma unroll 1
     for (int p = 0; p < kLdsProbes; p++) {
@@ -297,7 +107,7 @@ __device__ __forceinline__ void lds_accum(const Lds& t, int s, const Part& p, ui
     }
 }
 
-// ---- the row kernel ----------------------------------------------------------------------
+// ---- the one-pass row kernel --------------------------------------------------------------
 // MODE 0: every row; 1: rows whose int8 mask is non-zero; 2: rows passing the predicates.
 // Row r of tile T is T*(kSteps*blockDim) + k*blockDim + threadIdx.x, so each step's loads
 // are coalesced spans.  Workgroups are persistent (grid-stride over tiles) so that the LDS
@@ -310,10 +120,12 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
                                                   uint32_t vpred, int64_t n, uint32_t fields,
                                                   int32_t H) {
     extern __shared__ uint64_t lds_raw[];
+    __shared__ unsigned int s_new;
     constexpr bool HAS_V = VTraits<VT>::has;
     constexpr bool REAL = VTraits<VT>::real;
     const bool use_lds = (KW == 1) && H > 0;
     Lds t{};
+    if (threadIdx.x == 0) s_new = 0;
     if (use_lds) {
         t = lds_carve(lds_raw, H, fields);
         for (int s = threadIdx.x; s < H + 2; s += blockDim.x) {
@@ -324,8 +136,8 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
             if (t.mn) t.mn[s] = ~0ull;
             if (t.mx) t.mx[s] = 0ull;
         }
-        __syncthreads();
     }
+    __syncthreads();
 
     const int64_t bd = blockDim.x;
     const int64_t tile_rows = bd * kSteps;
@@ -340,13 +152,10 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
             act[k] = r < n;
             vn[k] = false;
             kn[k] = false;
+            val[k] = {};
             if (act[k]) {
                 load_key<KW>(ks, kc, r, key[k], kn[k]);
-                if constexpr (HAS_V) {
-                    val[k] = vals[r];
-                } else {
-                    val[k] = 0;
-                }
+                if constexpr (HAS_V) val[k] = vals[r];
                 vn[k] = vnull && null_bit(vnull, r);
                 if (MODE == 1) act[k] = mask[r] != 0;
             }
@@ -374,37 +183,22 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
         for (int k = 0; k < kSteps; k++) {
             if (!act[k]) continue;
             Part p;
-            p.cntall = 1;
-            p.cnt = vn[k] ? 0 : 1;
-            p.sum = 0;
-            p.mn = ~0ull;
-            p.mx = 0;
-            if constexpr (HAS_V) {
-                if (!vn[k]) {
-                    if constexpr (REAL) {
-                        const double d = (double)val[k];
-                        p.sum = (uint64_t)__double_as_longlong(d);
-                        p.mn = p.mx = enc_f64(d);
-                    } else {
-                        const int64_t iv = (int64_t)val[k];
-                        p.sum = (uint64_t)iv;
-                        p.mn = p.mx = enc_i64(iv);
-                    }
-                }
-            }
+            if constexpr (HAS_V) p = row_part<VT>(val[k], vn[k]); else p = row_part<NoValue>(NoValue{}, vn[k]);
             const uint64_t h = key_hash<KW>(key[k]);
+            bool is_new = false;
             if constexpr (KW == 1) {
                 int ls = use_lds ? lds_find(t, key[k][0], kn[k], h) : -1;
                 if (ls >= 0) {
                     lds_accum<REAL>(t, ls, p, fields);
                 } else {
-                    const int64_t gs = g_slot1(g, key[k][0], kn[k], h);
-                    if (gs >= 0) g_accum(g, gs, p, fields); else g_fail(g);
+                    const int64_t gs = g_slot1(g, key[k][0], kn[k], h, &is_new);
+                    if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
                 }
             } else {
-                const int64_t gs = g_slotN<KW>(g, key[k], h);
-                if (gs >= 0) g_accum(g, gs, p, fields); else g_fail(g);
+                const int64_t gs = g_slotN<KW>(g, key[k], h, &is_new);
+                if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
             }
+            if (is_new) atomicAdd(&s_new, 1u);
         }
     }
 
@@ -419,31 +213,34 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
             p.sum = t.sum ? t.sum[s] : 0;
             p.mn = t.mn ? t.mn[s] : ~0ull;
             p.mx = t.mx ? t.mx[s] : 0ull;
+            bool is_new = false;
             int64_t gs;
             if (s == H) {
-                gs = g_slot1(g, 0ull, true, 0);
+                gs = g_slot1(g, 0ull, true, 0, &is_new);
             } else if (s == H + 1) {
-                gs = g_slot1(g, kEmpty, false, 0);
+                gs = g_slot1(g, kEmpty, false, 0, &is_new);
             } else {
-                gs = g_slot1(g, k, false, hash64(k));
+                gs = g_slot1(g, k, false, hash64(k), &is_new);
             }
-            if (gs >= 0) g_accum(g, gs, p, fields); else g_fail(g);
+            if (is_new) atomicAdd(&s_new, 1u);
+            if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) g_count_new(g, s_new);
 }
 
 // ---- table initialisation ----------------------------------------------------------------
-__global__ void k_gb_init(GView g, int32_t kw) {
+__global__ void k_gb_init(GView g, uint32_t fields) {
     const uint64_t total = g.cap + 2;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += stride) {
-        uint64_t* r = rec_ptr(g, s);
-        for (int w = 0; w < kw; w++) r[w] = (kw == 1 && s < g.cap) ? kEmpty : 0ull;
-        r[g.rec.cnt] = 0;
-        if (g.rec.cntall >= 0) r[g.rec.cntall] = 0;
-        if (g.rec.sum >= 0) r[g.rec.sum] = 0;
-        if (g.rec.mn >= 0) r[g.rec.mn] = ~0ull;
-        if (g.rec.mx >= 0) r[g.rec.mx] = 0;
+        for (int w = 0; w < g.kw; w++) g.keys[s * g.kw + w] = (g.kw == 1 && s < g.cap) ? kEmpty : 0ull;
+        g.cnt[s] = 0;
+        if (g.cntall) g.cntall[s] = 0;
+        if (g.sum) g.sum[s] = 0;
+        if (g.mn) g.mn[s] = ~0ull;
+        if (g.mx) g.mx[s] = 0;
     }
 }
 
@@ -459,10 +256,10 @@ struct OutSpec {
     uint64_t* konull[kMaxKeys];
 };
 
-__device__ __forceinline__ bool slot_used(const GView& g, int32_t kw, uint64_t s) {
-    const uint64_t* r = rec_ptr(g, s);
-    if (kw == 1) return s < g.cap ? (r[0] != kEmpty) : (r[0] != 0ull);
-    return s < g.cap && r[kw - 1] != 0ull;
+__device__ __forceinline__ bool slot_used(const GView& g, uint64_t s) {
+    const uint64_t* k = g.keys + s * (uint64_t)g.kw;
+    if (g.kw == 1) return s < g.cap ? (k[0] != kEmpty) : (k[0] != 0ull);
+    return s < g.cap && k[g.kw - 1] != 0ull;
 }
 
 __device__ __forceinline__ void put_typed(void* out, int type, int64_t pos, int64_t iv, double dv,
@@ -480,10 +277,10 @@ __device__ __forceinline__ void set_bit(uint64_t* m, int64_t pos) {
     atomicOr((unsigned long long*)(m + (pos >> 6)), 1ull << (pos & 63));
 }
 
-// Key column c of the group in slot s.
+// Key columns of the group in slot s.
 __device__ __forceinline__ void emit_key(const GView& g, const KeySpec& ks, const OutSpec& os,
                                          uint64_t s, int64_t pos) {
-    const uint64_t* r = rec_ptr(g, s);
+    const uint64_t* r = g.keys + s * (uint64_t)g.kw;
     for (int c = 0; c < ks.nkeys; c++) {
         const int type = ks.type[c];
         bool isnull = false;
@@ -516,8 +313,7 @@ __device__ __forceinline__ void emit_key(const GView& g, const KeySpec& ks, cons
 }
 
 __device__ __forceinline__ void emit_aggs(const GView& g, const OutSpec& os, uint64_t s, int64_t pos) {
-    const uint64_t* r = rec_ptr(g, s);
-    const uint64_t cnt = r[g.rec.cnt];
+    const uint64_t cnt = g.cnt[s];
     const int vt = os.vtype;
     const bool vreal = vt == QK_FLOAT || vt == QK_DOUBLE;
     for (int a = 0; a < os.naggs; a++) {
@@ -528,17 +324,17 @@ __device__ __forceinline__ void emit_aggs(const GView& g, const OutSpec& os, uin
         bool isnull = false;
         switch (ag) {
         case QK_AGG_COUNT: ((int64_t*)o)[pos] = (int64_t)cnt; break;
-        case QK_AGG_COUNT_ALL: ((int64_t*)o)[pos] = (int64_t)r[g.rec.cntall]; break;
+        case QK_AGG_COUNT_ALL: ((int64_t*)o)[pos] = (int64_t)g.cntall[s]; break;
         case QK_AGG_SUM: {
             isnull = cnt == 0;
-            if (vreal) put_typed(o, ot, pos, 0, isnull ? 0.0 : __longlong_as_double((long long)r[g.rec.sum]), true);
-            else put_typed(o, ot, pos, isnull ? 0 : (int64_t)r[g.rec.sum], 0.0, false);
+            if (vreal) put_typed(o, ot, pos, 0, isnull ? 0.0 : __longlong_as_double((long long)g.sum[s]), true);
+            else put_typed(o, ot, pos, isnull ? 0 : (int64_t)g.sum[s], 0.0, false);
             break;
         }
         case QK_AGG_MIN:
         case QK_AGG_MAX: {
             isnull = cnt == 0;
-            const uint64_t e = r[ag == QK_AGG_MIN ? g.rec.mn : g.rec.mx];
+            const uint64_t e = ag == QK_AGG_MIN ? g.mn[s] : g.mx[s];
             if (vreal) {
                 double d = isnull ? (ag == QK_AGG_MIN ? (vt == QK_FLOAT ? 3.4028234663852886e38 : 1.7976931348623157e308)
                                                       : (vt == QK_FLOAT ? -3.4028234663852886e38 : -1.7976931348623157e308))
@@ -558,11 +354,11 @@ __device__ __forceinline__ void emit_aggs(const GView& g, const OutSpec& os, uin
             isnull = cnt == 0;
             if (vreal) {
                 double d = isnull ? __longlong_as_double(0x7ff8000000000000ll)
-                                  : __longlong_as_double((long long)r[g.rec.sum]) / (double)cnt;
+                                  : __longlong_as_double((long long)g.sum[s]) / (double)cnt;
                 put_typed(o, ot, pos, 0, d, true);
             } else {
                 int64_t v;
-                const uint64_t sm = r[g.rec.sum];
+                const uint64_t sm = g.sum[s];
                 if (vt == QK_INT) v = isnull ? INT32_MIN : (int64_t)(int32_t)((int64_t)(int32_t)(uint32_t)sm / (int64_t)cnt);
                 else if (vt == QK_INT8) v = isnull ? INT8_MIN : (int64_t)(int8_t)((int64_t)(int8_t)(uint8_t)sm / (int64_t)cnt);
                 else v = isnull ? INT64_MIN : (int64_t)sm / (int64_t)cnt;
@@ -594,7 +390,7 @@ __global__ void __launch_bounds__(kEmitBlock) k_gb_emit(GView g, KeySpec ks, Out
 #pragma unroll
     for (int k = 0; k < kEmitSteps; k++) {
         const uint64_t s = (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid;
-        const bool sel = s < nslots && slot_used(g, ks.kw, s);
+        const bool sel = s < nslots && slot_used(g, s);
         const uint64_t b = __ballot(sel);
         pre[k] = (uint32_t)__popcll(b & lt_mask);
         if (lane == 0) s_cnt[k][wid] = (int32_t)__popcll(b);
@@ -638,12 +434,8 @@ int launch_rows_t(const GView& g, const KeySpec& ks, const KeyCols& kc, const vo
                   int64_t n, uint32_t fields, int32_t H, int block, int wgs_per_cu,
                   hipStream_t s) {
     auto kern = k_gb_rows<KW, VT, MODE>;
+    // gfx950 launches up to 160 KB of LDS per workgroup without a function attribute
     const size_t lds = (KW == 1 && H > 0) ? (size_t)(H + 2) * lds_slot_bytes(fields) : 0;
-    static std::once_flag once;
-    std::call_once(once, [&] {
-        (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)device_info().lds_per_block);
-    });
     int occ = 0;
     QK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, lds));
     if (occ < 1) return fail(QK_ERROR, "group by: launch shape (block %d, LDS %zu B) does not fit", block, lds);
@@ -697,17 +489,23 @@ struct qk_groupby_s {
     int32_t aggs[kMaxAggs] = {}, out_types[kMaxAggs] = {};
     uint32_t flags = 0, fields = 0;
     KeySpec ks{};
-    Rec rec{};
     uint64_t logical_cap = 0;
     int32_t reported_max = 0;
     uint64_t cap = 0;
-    uint64_t* slots = nullptr;
-    TableCounters* ctr = nullptr;
+    int32_t parts = 1;
     bool real_sum = false;
-    // launch tuning
+    // storage (structure of arrays)
+    uint64_t* keys = nullptr;
+    unsigned long long *cnt = nullptr, *cntall = nullptr, *sum = nullptr, *mn = nullptr, *mx = nullptr;
+    TableCounters* ctr = nullptr;
+    // two-phase path
+    TwoPhaseBufs tp;
+    int64_t twophase_min_rows = (int64_t)1 << 20;
+    int32_t twophase_mode = -1;  // -1 automatic, 0 never, 1 always (single-word keys)
+    // launch tuning of the one-pass kernel
     int32_t block = 512;
     int32_t lds_budget = -1;  // bytes of LDS per workgroup; -1 = half of a CU's LDS, 0 = none
-    int32_t wgs_per_cu = 0;  // 0 = as many as fit
+    int32_t wgs_per_cu = 0;   // 0 = as many as fit
     int32_t last_H = 0;
     // profiling
     bool profiling = false;
@@ -722,13 +520,25 @@ namespace {
 
 GView view_of(const qk_groupby_s* h) {
     GView g;
-    g.slots = h->slots;
+    g.keys = h->keys;
+    g.cnt = h->cnt;
+    g.cntall = h->cntall;
+    g.sum = h->sum;
+    g.mn = h->mn;
+    g.mx = h->mx;
     g.cap = h->cap;
-    g.rec = h->rec;
+    int pbits = 0;
+    while ((1 << pbits) < h->parts) pbits++;
+    int cbits = 0;
+    while (((uint64_t)1 << cbits) < h->cap) cbits++;
+    g.part_shift = pbits == 0 ? 64 : 64 - pbits;
+    g.sub_bits = cbits - pbits;
+    g.sub_mask = ((uint64_t)1 << g.sub_bits) - 1;
+    g.kw = h->ks.kw;
+    g.real_sum = h->real_sum ? 1 : 0;
     g.nkeys = &h->ctr->nkeys;
     g.err = &h->ctr->err;
     g.logical_cap = h->logical_cap;
-    g.real_sum = h->real_sum ? 1 : 0;
     return g;
 }
 
@@ -742,9 +552,10 @@ int init_table(qk_groupby_s* h, hipStream_t s) {
     int64_t grid = (int64_t)((total + 255) / 256);
     const int64_t capg = (int64_t)device_info().cus * 16;
     if (grid > capg) grid = capg;
-    k_gb_init<<<(unsigned)grid, 256, 0, s>>>(g, h->ks.kw);
+    k_gb_init<<<(unsigned)grid, 256, 0, s>>>(g, h->fields);
     QK_HIP(hipGetLastError());
     QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
+    h->tp.heavy_ready = false;
     return QK_SUCCESS;
 }
 
@@ -785,6 +596,15 @@ int choose_lds_slots(const qk_groupby_s* h) {
     return (int)slots;
 }
 
+std::pair<hipEvent_t, hipEvent_t>* next_events(qk_groupby h) {
+    if (h->events_used == h->events.size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return nullptr;
+        h->events.emplace_back(a, b);
+    }
+    return &h->events[h->events_used++];
+}
+
 int process_common(qk_groupby h, int mode, const void* const* keys, const uint64_t* const* key_nulls,
                    const void* values, const uint64_t* value_nulls, const int8_t* mask,
                    int32_t npreds, const qk_predicate* preds, int32_t n, void* stream) {
@@ -812,29 +632,31 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
             ps.p[p].null_mask == value_nulls)
             vpred |= 1u << p;
     hipStream_t s = as_stream(stream);
-    const int32_t H = choose_lds_slots(h);
-    h->last_H = H;
     GView g = view_of(h);
-    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-    if (h->profiling) {
-        if (h->events_used == h->events.size()) {
-            hipEvent_t a, b;
-            QK_HIP(hipEventCreate(&a));
-            QK_HIP(hipEventCreate(&b));
-            h->events.emplace_back(a, b);
-        }
-        ev = &h->events[h->events_used++];
-        QK_HIP(hipEventRecord(ev->first, s));
-    }
+    std::pair<hipEvent_t, hipEvent_t>* ev = h->profiling ? next_events(h) : nullptr;
     const int vtype = values ? h->value_type : 0;
+    const bool two = twophase_supported(h->ks.kw, h->value_type) &&
+                     (h->twophase_mode == 1 || (h->twophase_mode < 0 && n >= h->twophase_min_rows));
     int st;
-    switch (h->ks.kw) {
-    case 1: st = launch_rows_v<1>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, H, h->block, h->wgs_per_cu, s); break;
-    case 2: st = launch_rows_v<2>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, 0, h->block, h->wgs_per_cu, s); break;
-    default: st = launch_rows_v<4>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, 0, h->block, h->wgs_per_cu, s); break;
+    if (two) {
+        QK_TRY(twophase_prepare(h->tp, g, h->parts, h->fields, h->value_type, values != nullptr, n, s));
+        if (!h->tp.heavy_ready)
+            QK_TRY(twophase_sample(h->tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n, mode, s));
+        h->last_H = h->tp.H1;
+        st = twophase_block(h->tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n,
+                            h->fields, mode, s, ev ? ev->first : nullptr, ev ? ev->second : nullptr);
+    } else {
+        const int32_t H = choose_lds_slots(h);
+        h->last_H = H;
+        if (ev) QK_HIP(hipEventRecord(ev->first, s));
+        switch (h->ks.kw) {
+        case 1: st = launch_rows_v<1>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, H, h->block, h->wgs_per_cu, s); break;
+        case 2: st = launch_rows_v<2>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, 0, h->block, h->wgs_per_cu, s); break;
+        default: st = launch_rows_v<4>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, 0, h->block, h->wgs_per_cu, s); break;
+        }
+        if (st == QK_SUCCESS && ev) QK_HIP(hipEventRecord(ev->second, s));
     }
     if (st != QK_SUCCESS) return st;
-    if (ev) QK_HIP(hipEventRecord(ev->second, s));
     if (!(h->flags & QK_GB_ASYNC)) return take_error(h, s);
     return QK_SUCCESS;
 }
@@ -849,6 +671,13 @@ int collect_profile(qk_groupby h) {
     }
     h->events_used = 0;
     return QK_SUCCESS;
+}
+
+void free_table(qk_groupby_s* h) {
+    void* ptrs[] = {h->keys, h->cnt, h->cntall, h->sum, h->mn, h->mx, h->ctr};
+    for (void* p : ptrs)
+        if (p) pool_free(p);
+    twophase_free(h->tp);
 }
 
 }  // namespace
@@ -890,6 +719,9 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     if (needs_value && value_type != QK_INT && value_type != QK_LONG && value_type != QK_FLOAT &&
         value_type != QK_DOUBLE)
         return fail(QK_ERROR, "qk_gb_create: value type %d unsupported", value_type);
+    if (!needs_value && value_type != QK_INT && value_type != QK_LONG && value_type != QK_FLOAT &&
+        value_type != QK_DOUBLE)
+        value_type = h->value_type = QK_INT;  // no value column is read; any placeholder type
     for (int a = 0; a < naggs; a++) {
         const int ag = aggs[a];
         int dflt = (ag == QK_AGG_COUNT || ag == QK_AGG_COUNT_ALL) ? QK_LONG : value_type;
@@ -912,7 +744,6 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     // key layout
     KeySpec ks{};
     ks.nkeys = nkeys;
-    int total_bits = 0;
     for (int c = 0; c < nkeys; c++) {
         const int t = key_types[c];
         if (!type_size(t)) return fail(QK_ERROR, "qk_gb_create: key type %d unsupported", t);
@@ -931,8 +762,7 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
             ks.shift[c] = pos % 64;
             pos += w;
         }
-        total_bits = pos;
-        if ((flags & QK_GB_KEYS_NOT_NULL) && total_bits <= 64) {
+        if ((flags & QK_GB_KEYS_NOT_NULL) && pos <= 64) {
             ks.kw = 1;
         } else {
             ks.nullable = (flags & QK_GB_KEYS_NOT_NULL) ? 0 : 1;
@@ -943,27 +773,12 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
                     pos++;
                 }
             const int kw = (pos + 1 + 63) / 64;  // + the present bit (bit 63 of the last word)
-            if (ks.nullable) {
-                // the null flags must not collide with the present bit
-                if (pos > kw * 64 - 1) return fail(QK_ERROR, "qk_gb_create: key too wide");
-            }
+            if (pos > kw * 64 - 1) return fail(QK_ERROR, "qk_gb_create: key too wide");
             if (kw > 4) return fail(QK_ERROR, "qk_gb_create: composite key wider than 255 bits");
             ks.kw = kw == 3 ? 4 : kw;
         }
     }
     h->ks = ks;
-
-    // record layout
-    Rec r{};
-    int w = ks.kw;
-    r.kw = ks.kw;
-    r.cnt = w++;
-    r.cntall = (fields & F_CNTALL) ? w++ : -1;
-    r.sum = (fields & F_SUM) ? w++ : -1;
-    r.mn = (fields & F_MIN) ? w++ : -1;
-    r.mx = (fields & F_MAX) ? w++ : -1;
-    r.words = w;
-    h->rec = r;
 
     const uint64_t gmax = (uint64_t)1 << 30;  // GB_BUFFER_SIZE_MAX (IGroupBy.h:6)
     uint64_t logical = std::min<uint64_t>((uint64_t)max_hash_count * (uint64_t)key_size_multiplier, gmax);
@@ -972,21 +787,34 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     uint64_t cap = 64;
     while (cap < 2 * logical) cap <<= 1;
     h->cap = cap;
-    QK_TRY(pool_alloc((void**)&h->slots, (size_t)(cap + 2) * r.words * 8));
-    int st = pool_alloc((void**)&h->ctr, sizeof(TableCounters));
-    if (st != QK_SUCCESS) {
-        pool_free(h->slots);
-        return st;
+    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only)
+    int32_t parts = 1;
+    if (ks.kw == 1)
+        while (parts < 256 && cap / (uint64_t)(parts * 2) >= 1024) parts *= 2;
+    h->parts = parts;
+    const size_t slots = (size_t)(cap + 2);
+    int st = QK_SUCCESS;
+    auto A = [&](void** p, size_t bytes) {
+        if (st == QK_SUCCESS) st = pool_alloc(p, bytes);
+    };
+    A((void**)&h->keys, slots * ks.kw * 8);
+    A((void**)&h->cnt, slots * 8);
+    if (fields & F_CNTALL) A((void**)&h->cntall, slots * 8);
+    if (fields & F_SUM) A((void**)&h->sum, slots * 8);
+    if (fields & F_MIN) A((void**)&h->mn, slots * 8);
+    if (fields & F_MAX) A((void**)&h->mx, slots * 8);
+    A((void**)&h->ctr, sizeof(TableCounters));
+    if (st == QK_SUCCESS) {
+        hipError_t e = hipGetDevice(&h->device);
+        if (e != hipSuccess) st = fail(QK_RUNTIME_ERROR, "hipGetDevice: %s", hipGetErrorString(e));
     }
-    QK_HIP(hipGetDevice(&h->device));
-    st = init_table(h.get(), nullptr);
+    if (st == QK_SUCCESS) st = init_table(h.get(), nullptr);
     if (st == QK_SUCCESS) {
         hipError_t e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) st = fail(QK_RUNTIME_ERROR, "table init: %s", hipGetErrorString(e));
     }
     if (st != QK_SUCCESS) {
-        pool_free(h->slots);
-        pool_free(h->ctr);
+        free_table(h.get());
         return st;
     }
     *out = h.release();
@@ -996,8 +824,7 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
 int qk_gb_destroy(qk_groupby h) {
     if (!h) return QK_SUCCESS;
     (void)hipDeviceSynchronize();
-    pool_free(h->slots);
-    pool_free(h->ctr);
+    free_table(h);
     for (auto& e : h->events) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -1049,13 +876,13 @@ int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream) {
     QK_TRY(read_counters(h, s, &c));
     uint64_t side = 0;
     if (h->ks.kw == 1) {
-        uint64_t w[1];
         void* hb = nullptr;
         QK_TRY(pinned_host(8, &hb));
-        QK_HIP(hipMemcpyAsync(hb, h->slots + h->cap * h->rec.words, 8, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipMemcpyAsync(hb, h->keys + h->cap, 8, hipMemcpyDeviceToHost, s));
         QK_HIP(hipStreamSynchronize(s));
-        memcpy(w, hb, 8);
-        side = w[0] != 0 ? 1 : 0;  // the NULL group
+        uint64_t w;
+        memcpy(&w, hb, 8);
+        side = w != 0 ? 1 : 0;  // the NULL group
     }
     *out = (int64_t)(c.nkeys + side);
     return QK_SUCCESS;
@@ -1181,7 +1008,14 @@ int qk_gb_get_profile(qk_groupby h, double* ms, int64_t* launches) {
 
 int qk_gb_describe(qk_groupby h, int32_t* slot_bytes, int64_t* slots, int32_t* lds_slots) {
     if (!h) return fail(QK_ERROR, "qk_gb_describe: null table");
-    if (slot_bytes) *slot_bytes = h->rec.words * 8;
+    if (slot_bytes) {
+        int b = h->ks.kw * 8 + 8;
+        if (h->cntall) b += 8;
+        if (h->sum) b += 8;
+        if (h->mn) b += 8;
+        if (h->mx) b += 8;
+        *slot_bytes = b;
+    }
     if (slots) *slots = (int64_t)h->cap;
     if (lds_slots) *lds_slots = h->last_H;
     return QK_SUCCESS;
@@ -1195,6 +1029,14 @@ int qk_gb_set_tuning(qk_groupby h, int32_t block, int32_t lds_bytes, int32_t wgs
     }
     if (lds_bytes >= 0) h->lds_budget = lds_bytes;
     if (wgs_per_cu >= 0) h->wgs_per_cu = wgs_per_cu;
+    return QK_SUCCESS;
+}
+
+int qk_gb_set_path(qk_groupby h, int32_t path, int64_t min_rows) {
+    if (!h) return fail(QK_ERROR, "qk_gb_set_path: null table");
+    if (path < -1 || path > 1) return fail(QK_ERROR, "qk_gb_set_path: path must be -1, 0 or 1");
+    h->twophase_mode = path;
+    if (min_rows >= 0) h->twophase_min_rows = min_rows;
     return QK_SUCCESS;
 }
 

@@ -1,0 +1,789 @@
+// twophase.hip - two-phase partitioned GROUP BY for single-word keys (see twophase.hpp).
+//
+// Heavy hitters: a strided sample of the first block is counted in a device hash table; the
+// keys seen at least `t` times (t chosen from the count histogram so that at most kmax keys
+// qualify) become the heavy set, sorted by partition.  All of it runs on the device.
+#include <algorithm>
+
+#include "twophase.hpp"
+
+namespace qk {
+
+namespace {
+
+constexpr int kP1Steps = 4;   // rows per thread per tile in phase 1
+constexpr int kBlock = 1024;
+
+// VV is VT, or a placeholder type when the table reads no value column.
+template <typename VT, int MODE, typename VV>
+__device__ __forceinline__ bool row_passes(const PredSet& preds, uint32_t vpred, const int8_t* mask,
+                                           int64_t r, const VV& v, bool vn) {
+    if (MODE == 1) return mask[r] != 0;
+    if (MODE == 2) {
+        for (int p = 0; p < preds.n; p++) {
+            const qk_predicate& q = preds.p[p];
+            bool pass;
+            if constexpr (VTraits<VT>::has) {
+                if ((vpred >> p) & 1u)
+                    pass = !vn && pred_on_value(q, v, r);
+                else
+                    pass = eval_pred(q, r);
+            } else {
+                pass = eval_pred(q, r);
+            }
+            if (!pass) return false;
+        }
+    }
+    return true;
+}
+
+// ---- sampling --------------------------------------------------------------------------
+
+__global__ void k_tp_clear_sample(uint64_t* keys, uint32_t* cnt, uint64_t slots, uint32_t* hist,
+                                  int32_t* meta) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride) {
+        keys[s] = kEmpty;
+        cnt[s] = 0;
+    }
+    if (blockIdx.x == 0) {
+        for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) hist[b] = 0;
+        if (threadIdx.x == 0) meta[0] = 0;
+    }
+}
+
+template <typename VT, int MODE>
+__global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const VT* __restrict__ vals,
+                                                   const uint64_t* __restrict__ vnull,
+                                                   const int8_t* __restrict__ mask, PredSet preds,
+                                                   uint32_t vpred, int64_t n, int64_t stride,
+                                                   int64_t nsamples, uint64_t* skeys,
+                                                   uint32_t* scnt, uint64_t smask) {
+    const int64_t gs = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsamples; i += gs) {
+        const int64_t r = i * stride;
+        if (r >= n) break;
+        typename std::conditional<VTraits<VT>::has, VT, int>::type v{};
+        if constexpr (VTraits<VT>::has) v = vals[r];
+        const bool vn = vnull && null_bit(vnull, r);
+        if (!row_passes<VT, MODE>(preds, vpred, mask, r, v, vn)) continue;
+        uint64_t k[1];
+        bool kn;
+        load_key<1>(ks, kc, r, k, kn);
+        if (kn || k[0] == kEmpty) continue;  // those keys have side slots anyway
+        uint64_t s = hash64(k[0]) & smask;
+        for (uint64_t p = 0; p <= smask; p++) {
+            uint64_t cur = *(volatile uint64_t*)(skeys + s);
+            if (cur == kEmpty) {
+                cur = atomicCAS((unsigned long long*)(skeys + s), (unsigned long long)kEmpty,
+                                (unsigned long long)k[0]);
+                if (cur == kEmpty) cur = k[0];
+            }
+            if (cur == k[0]) {
+                atomicAdd(scnt + s, 1u);
+                break;
+            }
+            s = (s + 1) & smask;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tp_hist(const uint64_t* __restrict__ skeys,
+                                                 const uint32_t* __restrict__ scnt, uint64_t slots,
+                                                 uint32_t* hist) {
+    __shared__ uint32_t lh[kHistBins];
+    for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) lh[b] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride)
+        if (skeys[s] != kEmpty) atomicAdd(&lh[min(scnt[s], (uint32_t)(kHistBins - 1))], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kHistBins; b += blockDim.x)
+        if (lh[b]) atomicAdd(hist + b, lh[b]);
+}
+
+// Smallest count t >= 2 such that at most kmax sampled keys were seen t times or more.
+__global__ void k_tp_threshold(const uint32_t* hist, int32_t kmax, int32_t* meta) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t cum = 0;
+    int32_t t = kHistBins;
+    for (int b = kHistBins - 1; b >= 2; b--) {
+        if (cum + hist[b] > (uint32_t)kmax) break;
+        cum += hist[b];
+        t = b;
+    }
+    meta[1] = t;
+    meta[0] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_tp_collect(const uint64_t* __restrict__ skeys,
+                                                    const uint32_t* __restrict__ scnt, uint64_t slots,
+                                                    int32_t kmax, int32_t* meta, uint64_t* cand) {
+    const uint32_t t = (uint32_t)meta[1];
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride) {
+        if (skeys[s] != kEmpty && scnt[s] >= t) {
+            int32_t i = atomicAdd(meta, 1);
+            if (i < kmax) cand[i] = skeys[s];
+        }
+    }
+}
+
+// One workgroup: order the heavy candidates by partition; meta[2 + p] = first index of p.
+__global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ cand, int32_t kmax,
+                                                  int32_t P, int32_t part_shift, int32_t* meta,
+                                                  uint64_t* heavy) {
+    __shared__ int32_t cnt[1024];
+    __shared__ int32_t cur[1024];
+    const int t = threadIdx.x;
+    const int32_t K = min(meta[0], kmax);
+    for (int p = t; p < P; p += blockDim.x) cnt[p] = 0;
+    __syncthreads();
+    for (int i = t; i < K; i += blockDim.x) {
+        const uint64_t h = hash64(cand[i]);
+        const int p = part_shift >= 64 ? 0 : (int)(h >> part_shift);
+        atomicAdd(&cnt[p], 1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        int32_t run = 0;
+        for (int p = 0; p < P; p++) {
+            cur[p] = run;
+            meta[2 + p] = run;
+            run += cnt[p];
+        }
+        meta[2 + P] = run;
+        meta[0] = K;
+    }
+    __syncthreads();
+    for (int i = t; i < K; i += blockDim.x) {
+        const uint64_t h = hash64(cand[i]);
+        const int p = part_shift >= 64 ? 0 : (int)(h >> part_shift);
+        heavy[atomicAdd(&cur[p], 1)] = cand[i];
+    }
+}
+
+// ---- phase 1 -------------------------------------------------------------------------------
+
+struct P1Args {
+    const uint64_t* heavy;
+    const int32_t* meta;
+    int32_t kmax, H, P, cap;
+    uint32_t* p_cnt;
+    uint32_t* p_rows;
+    uint64_t* p_sum;
+    uint64_t* p_min;
+    uint64_t* p_max;
+    uint64_t* r_keys;
+    void* r_vals;
+    uint32_t* r_count;
+};
+
+__host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, uint32_t fields) {
+    const size_t T = (size_t)H + 2;
+    size_t b = (size_t)H * 8;                                  // keys
+    b += T * 8 * (((fields & F_SUM) ? 1 : 0) + ((fields & F_MIN) ? 1 : 0) + ((fields & F_MAX) ? 1 : 0));
+    b += T * 4 * 2;                                            // cnt, rows
+    b += (size_t)H * 4;                                        // heavy index
+    b += (size_t)P * 4 + 16;                                   // cursors, counters
+    return b;
+}
+
+// LDS home slot of a key in a table of H slots (high hash bits, range-reduced).
+__device__ __forceinline__ uint32_t lds_home(uint64_t h, int32_t H) {
+    return (uint32_t)(((h >> 32) * (uint64_t)(uint32_t)H) >> 32);
+}
+
+template <typename VT, int MODE>
+__global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCols kc,
+                                                      const VT* __restrict__ vals,
+                                                      const uint64_t* __restrict__ vnull,
+                                                      const int8_t* __restrict__ mask, PredSet preds,
+                                                      uint32_t vpred, int64_t n, uint32_t fields,
+                                                      P1Args a) {
+    extern __shared__ uint64_t lds[];
+    constexpr bool HAS_V = VTraits<VT>::has;
+    constexpr bool REAL = VTraits<VT>::real;
+    const int H = a.H, T = H + 2, bd = blockDim.x, tid = threadIdx.x;
+    uint64_t* lkey = lds;
+    uint64_t* p = lkey + H;
+    uint64_t* lsum = (fields & F_SUM) ? p : nullptr;
+    p += (fields & F_SUM) ? T : 0;
+    uint64_t* lmin = (fields & F_MIN) ? p : nullptr;
+    p += (fields & F_MIN) ? T : 0;
+    uint64_t* lmax = (fields & F_MAX) ? p : nullptr;
+    p += (fields & F_MAX) ? T : 0;
+    uint32_t* lcnt = (uint32_t*)p;
+    uint32_t* lrows = lcnt + T;
+    int32_t* lidx = (int32_t*)(lrows + T);
+    uint32_t* cursor = (uint32_t*)(lidx + H);
+    uint32_t* nnew = cursor + a.P;
+
+    for (int s = tid; s < T; s += bd) {
+        if (s < H) {
+            lkey[s] = kEmpty;
+            lidx[s] = -1;
+        }
+        lcnt[s] = 0;
+        lrows[s] = 0;
+        if (lsum) lsum[s] = 0;
+        if (lmin) lmin[s] = ~0ull;
+        if (lmax) lmax[s] = 0;
+    }
+    for (int q = tid; q < a.P; q += bd) cursor[q] = 0;
+    if (tid == 0) nnew[0] = 0;
+    __syncthreads();
+    const int32_t K = min(a.meta[0], a.kmax);
+    for (int k = tid; k < K; k += bd) {
+        const uint64_t key = a.heavy[k];
+        uint32_t s = lds_home(hash64(key), H);
+        for (int it = 0; it < H; it++) {
+            unsigned long long old = atomicCAS((unsigned long long*)&lkey[s], (unsigned long long)kEmpty,
+                                               (unsigned long long)key);
+            if (old == kEmpty) {
+                lidx[s] = k;
+                break;
+            }
+            if (old == key) break;
+            s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
+        }
+    }
+    __syncthreads();
+
+    const int64_t tile_rows = (int64_t)bd * kP1Steps;
+    const int64_t ntiles = (n + tile_rows - 1) / tile_rows;
+    const int64_t region0 = (int64_t)blockIdx.x * a.P;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint64_t key[kP1Steps][1];
+        bool kn[kP1Steps], act[kP1Steps], vn[kP1Steps];
+        typename std::conditional<HAS_V, VT, int>::type val[kP1Steps];
+#pragma unroll
+        for (int k = 0; k < kP1Steps; k++) {
+            const int64_t r = tile * tile_rows + (int64_t)k * bd + tid;
+            act[k] = r < n;
+            vn[k] = false;
+            kn[k] = false;
+            key[k][0] = 0;
+            val[k] = {};
+            if (act[k]) {
+                load_key<1>(ks, kc, r, key[k], kn[k]);
+                if constexpr (HAS_V) val[k] = vals[r];
+                vn[k] = vnull && null_bit(vnull, r);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kP1Steps; k++) {
+            if (!act[k]) continue;
+            const int64_t r = tile * tile_rows + (int64_t)k * bd + tid;
+            if (!row_passes<VT, MODE>(preds, vpred, mask, r, val[k], vn[k])) continue;
+            const uint64_t kk = key[k][0];
+            const uint64_t h = hash64(kk);
+            int ls = -1;
+            if (kn[k]) {
+                ls = H;
+            } else if (kk == kEmpty) {
+                ls = H + 1;
+            } else {
+                uint32_t s = lds_home(h, H);
+                for (int it = 0; it < H; it++) {  // at most half full: a miss ends at an empty slot
+                    const uint64_t cur = lkey[s];
+                    if (cur == kk) {
+                        ls = (int)s;
+                        break;
+                    }
+                    if (cur == kEmpty) break;
+                    s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
+                }
+            }
+            Part pr;
+            if constexpr (HAS_V) pr = row_part<VT>(val[k], vn[k]); else pr = row_part<NoValue>(NoValue{}, vn[k]);
+            if (ls >= 0) {
+                atomicAdd(&lrows[ls], 1u);
+                if (pr.cnt) {
+                    atomicAdd(&lcnt[ls], 1u);
+                    if (lsum) {
+                        if (REAL)
+                            atomicAdd((double*)&lsum[ls], __longlong_as_double((long long)pr.sum));
+                        else
+                            atomicAdd((unsigned long long*)&lsum[ls], (unsigned long long)pr.sum);
+                    }
+                    if (lmin) atomicMin((unsigned long long*)&lmin[ls], (unsigned long long)pr.mn);
+                    if (lmax) atomicMax((unsigned long long*)&lmax[ls], (unsigned long long)pr.mx);
+                }
+                continue;
+            }
+            if (!vn[k]) {
+                const uint32_t q = part_of(g, h);
+                const uint32_t c = atomicAdd(&cursor[q], 1u);
+                if (c < (uint32_t)a.cap) {
+                    const int64_t at = (region0 + q) * (int64_t)a.cap + c;
+                    a.r_keys[at] = kk;
+                    if constexpr (HAS_V) ((VT*)a.r_vals)[at] = val[k];
+                    continue;
+                }
+            }
+            // NULL value or full region: atomic path
+            bool is_new = false;
+            const int64_t gs = g_slot1(g, kk, false, h, &is_new);
+            if (is_new) atomicAdd(nnew, 1u);
+            if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+        }
+    }
+
+    __syncthreads();
+    const int64_t pbase = (int64_t)blockIdx.x * (a.kmax + 2);
+    for (int s = tid; s < T; s += bd) {
+        int32_t k;
+        if (s < H) {
+            if (lkey[s] == kEmpty) continue;
+            k = lidx[s];
+        } else {
+            k = a.kmax + (s - H);
+        }
+        a.p_rows[pbase + k] = lrows[s];
+        a.p_cnt[pbase + k] = lcnt[s];
+        if (lsum) a.p_sum[pbase + k] = lsum[s];
+        if (lmin) a.p_min[pbase + k] = lmin[s];
+        if (lmax) a.p_max[pbase + k] = lmax[s];
+    }
+    for (int q = tid; q < a.P; q += bd) a.r_count[region0 + q] = min(cursor[q], (uint32_t)a.cap);
+    if (tid == 0) g_count_new(g, nnew[0]);
+}
+
+// ---- phase 2 -------------------------------------------------------------------------------
+
+struct P2Args {
+    const uint64_t* heavy;
+    const int32_t* meta;
+    int32_t kmax, H2, P, cap, W;
+    const uint32_t* p_cnt;
+    const uint32_t* p_rows;
+    const uint64_t* p_sum;
+    const uint64_t* p_min;
+    const uint64_t* p_max;
+    const uint64_t* r_keys;
+    const void* r_vals;
+    const uint32_t* r_count;
+};
+
+__host__ __device__ inline size_t p2_lds_bytes(int32_t H2, int32_t W, uint32_t fields) {
+    const size_t T = (size_t)H2 + 2;
+    size_t b = (size_t)H2 * 8;
+    b += T * 8 * (((fields & F_SUM) ? 1 : 0) + ((fields & F_MIN) ? 1 : 0) + ((fields & F_MAX) ? 1 : 0));
+    b += T * 4 * 2;
+    b += ((size_t)W + 1) * 4 + 16;
+    return b;
+}
+
+__device__ __forceinline__ uint32_t p2_home(uint64_t h, int32_t H2) {
+    return (uint32_t)((((h >> 8) & 0xffffffffull) * (uint64_t)(uint32_t)H2) >> 32);
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, P2Args a) {
+    extern __shared__ uint64_t lds[];
+    constexpr bool HAS_V = VTraits<VT>::has;
+    constexpr bool REAL = VTraits<VT>::real;
+    const int H = a.H2, T = H + 2, bd = blockDim.x, tid = threadIdx.x, part = blockIdx.x;
+    uint64_t* tkey = lds;
+    uint64_t* p = tkey + H;
+    uint64_t* tsum = (fields & F_SUM) ? p : nullptr;
+    p += (fields & F_SUM) ? T : 0;
+    uint64_t* tmin = (fields & F_MIN) ? p : nullptr;
+    p += (fields & F_MIN) ? T : 0;
+    uint64_t* tmax = (fields & F_MAX) ? p : nullptr;
+    p += (fields & F_MAX) ? T : 0;
+    uint32_t* tcnt = (uint32_t*)p;
+    uint32_t* trows = tcnt + T;
+    int32_t* roff = (int32_t*)(trows + T);
+    uint32_t* misc = (uint32_t*)(roff + a.W + 1);  // [0] used slots, [1] new keys
+
+    auto clear_slot = [&](int s) {
+        if (s < H) tkey[s] = kEmpty;
+        tcnt[s] = 0;
+        trows[s] = 0;
+        if (tsum) tsum[s] = 0;
+        if (tmin) tmin[s] = ~0ull;
+        if (tmax) tmax[s] = 0;
+    };
+    for (int s = tid; s < T; s += bd) clear_slot(s);
+    if (tid == 0) {
+        misc[0] = 0;
+        misc[1] = 0;
+    }
+    // prefix of the region sizes of this partition over the phase-1 workgroups
+    if (tid == 0) {
+        int32_t run = 0;
+        for (int w = 0; w < a.W; w++) {
+            roff[w] = run;
+            run += (int32_t)a.r_count[(int64_t)w * a.P + part];
+        }
+        roff[a.W] = run;
+    }
+    __syncthreads();
+
+    // The chunking below keeps the table at most 4/5 full, so a free slot always exists;
+    // the bound only guards against a broken invariant (reported as a full table).
+    auto insert = [&](uint64_t key) -> int {
+        uint32_t s = p2_home(hash64(key), H);
+        for (int it = 0; it < H; it++) {
+            const uint64_t cur = tkey[s];
+            if (cur == key) return (int)s;
+            if (cur == kEmpty) {
+                unsigned long long old = atomicCAS((unsigned long long*)&tkey[s], (unsigned long long)kEmpty,
+                                                   (unsigned long long)key);
+                if (old == kEmpty) {
+                    atomicAdd(&misc[0], 1u);
+                    return (int)s;
+                }
+                if (old == key) return (int)s;
+            }
+            s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
+        }
+        g_set_error(g, QK_HASH_TABLE_FULL);
+        return -1;
+    };
+    auto add = [&](int s, uint32_t rows, uint32_t cnt, uint64_t sum, uint64_t mn, uint64_t mx) {
+        if (rows) atomicAdd(&trows[s], rows);
+        if (cnt) {
+            atomicAdd(&tcnt[s], cnt);
+            if (tsum) {
+                if (REAL)
+                    atomicAdd((double*)&tsum[s], __longlong_as_double((long long)sum));
+                else
+                    atomicAdd((unsigned long long*)&tsum[s], (unsigned long long)sum);
+            }
+            if (tmin) atomicMin((unsigned long long*)&tmin[s], (unsigned long long)mn);
+            if (tmax) atomicMax((unsigned long long*)&tmax[s], (unsigned long long)mx);
+        }
+    };
+    auto part_of_slot = [&](int s) {
+        Part q;
+        q.cnt = tcnt[s];
+        q.cntall = trows[s];
+        q.sum = tsum ? tsum[s] : 0;
+        q.mn = tmin ? tmin[s] : ~0ull;
+        q.mx = tmax ? tmax[s] : 0;
+        return q;
+    };
+    // fold the LDS table into this partition's sub-table (this workgroup is its only writer)
+    auto merge = [&]() {
+        __syncthreads();
+        for (int s = tid; s < H; s += bd) {
+            const uint64_t key = tkey[s];
+            if (key == kEmpty) continue;
+            bool is_new = false;
+            const int64_t gs = g_slot1(g, key, false, hash64(key), &is_new);
+            if (is_new) atomicAdd(&misc[1], 1u);
+            if (gs >= 0) g_accum_owned(g, gs, part_of_slot(s), fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+            clear_slot(s);
+        }
+        __syncthreads();
+        if (tid == 0) misc[0] = 0;
+        __syncthreads();
+    };
+
+    // heavy partials of this partition's heavy keys, summed over the phase-1 workgroups
+    const int32_t hk0 = a.meta[2 + part], hk1 = a.meta[2 + part + 1];
+    const int64_t nh = (int64_t)(hk1 - hk0) * a.W;
+    const int32_t stride_k = a.kmax + 2;
+    for (int64_t j = tid; j < nh; j += bd) {
+        const int32_t k = hk0 + (int32_t)(j / a.W), w = (int32_t)(j % a.W);
+        const int64_t at = (int64_t)w * stride_k + k;
+        const uint32_t rows = a.p_rows[at];
+        if (!rows) continue;
+        const int s = insert(a.heavy[k]);
+        if (s < 0) continue;
+        add(s, rows, a.p_cnt[at], tsum ? a.p_sum[at] : 0, tmin ? a.p_min[at] : ~0ull,
+            tmax ? a.p_max[at] : 0);
+    }
+    if (part == 0) {  // NULL key and key == kEmpty: side slots
+        for (int64_t j = tid; j < 2 * (int64_t)a.W; j += bd) {
+            const int32_t side = (int32_t)(j / a.W), w = (int32_t)(j % a.W);
+            const int64_t at = (int64_t)w * stride_k + a.kmax + side;
+            const uint32_t rows = a.p_rows[at];
+            if (!rows) continue;
+            add(H + side, rows, a.p_cnt[at], tsum ? a.p_sum[at] : 0, tmin ? a.p_min[at] : ~0ull,
+                tmax ? a.p_max[at] : 0);
+        }
+    }
+
+    // cold rows of this partition, in chunks that cannot overfill the LDS table
+    const int64_t total = roff[a.W];
+    const int64_t limit = (int64_t)H * 4 / 5;
+    int64_t chunk = std::min<int64_t>((int64_t)bd * 4, limit);
+    if (chunk < 1) chunk = 1;
+    for (int64_t base = 0; base < total; base += chunk) {
+        __syncthreads();
+        // every thread reads the fill level before any thread inserts again
+        const bool full = (int64_t)misc[0] + chunk > limit;
+        __syncthreads();
+        if (full) merge();
+        const int64_t end = std::min<int64_t>(base + chunk, total);
+        for (int64_t j = base + tid; j < end; j += bd) {
+            // region of row j: last w with roff[w] <= j
+            int lo = 0, hi = a.W;  // roff[lo] <= j < roff[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if ((int64_t)roff[mid] <= j) lo = mid; else hi = mid;
+            }
+            const int64_t at = ((int64_t)lo * a.P + part) * (int64_t)a.cap + (j - roff[lo]);
+            const uint64_t key = a.r_keys[at];
+            const int s = insert(key);
+            if (s < 0) continue;
+            if constexpr (HAS_V) {
+                const Part pr = row_part<VT>(((const VT*)a.r_vals)[at], false);
+                add(s, 1, 1, pr.sum, pr.mn, pr.mx);
+            } else {
+                add(s, 1, 1, 0, ~0ull, 0);
+            }
+        }
+    }
+    merge();
+    if (part == 0 && tid < 2) {
+        const int s = H + tid;
+        if (trows[s]) {
+            const uint64_t gs = g.cap + tid;
+            if (g_claim_side(g, gs) && tid == 1) atomicAdd(&misc[1], 1u);
+            g_accum_owned(g, (int64_t)gs, part_of_slot(s), fields);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) g_count_new(g, misc[1]);
+}
+
+// ---- launch helpers ----------------------------------------------------------------------
+
+int grid_cap(int64_t items, int per, int cap) {
+    int64_t g = (items + per - 1) / per;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+template <typename VT, int MODE>
+int launch_sample(TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc, const void* vals,
+                  const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
+                  int64_t n, int64_t nsamples, int64_t stride, uint64_t smask, hipStream_t s) {
+    k_tp_sample<VT, MODE><<<grid_cap(nsamples, 256, device_info().cus * 4), 256, 0, s>>>(
+        ks, kc, (const VT*)vals, vnull, mask, ps, vpred, n, stride, nsamples, b.sample_keys,
+        b.sample_cnt, smask);
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+template <typename VT>
+int launch_sample_m(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc,
+                    const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
+                    uint32_t vpred, int64_t n, int64_t ns, int64_t st, uint64_t sm, hipStream_t s) {
+    switch (mode) {
+    case 0: return launch_sample<VT, 0>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+    case 1: return launch_sample<VT, 1>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+    default: return launch_sample<VT, 2>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+    }
+}
+
+template <typename VT, int MODE>
+int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc, const void* vals,
+              const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
+              int64_t n, uint32_t fields, hipStream_t s) {
+    auto kern = k_tp_phase1<VT, MODE>;  // gfx950 launches up to 160 KB of LDS without an attribute
+    P1Args a;
+    a.heavy = b.heavy_keys;
+    a.meta = b.heavy_meta;
+    a.kmax = b.kmax;
+    a.H = b.H1;
+    a.P = b.P;
+    a.cap = b.cap;
+    a.p_cnt = b.p_cnt;
+    a.p_rows = b.p_rows;
+    a.p_sum = b.p_sum;
+    a.p_min = b.p_min;
+    a.p_max = b.p_max;
+    a.r_keys = b.r_keys;
+    a.r_vals = b.r_vals;
+    a.r_count = b.r_count;
+    const size_t lds = p1_lds_bytes(b.H1, b.P, fields);
+    kern<<<b.W, b.block, lds, s>>>(g, ks, kc, (const VT*)vals, vnull, mask, ps, vpred, n, fields, a);
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+template <typename VT>
+int launch_p1_m(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
+                uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
+    switch (mode) {
+    case 0: return launch_p1<VT, 0>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 1: return launch_p1<VT, 1>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    default: return launch_p1<VT, 2>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    }
+}
+
+template <typename VT>
+int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
+    auto kern = k_tp_phase2<VT>;
+    P2Args a;
+    a.heavy = b.heavy_keys;
+    a.meta = b.heavy_meta;
+    a.kmax = b.kmax;
+    a.H2 = b.H2;
+    a.P = b.P;
+    a.cap = b.cap;
+    a.W = b.W;
+    a.p_cnt = b.p_cnt;
+    a.p_rows = b.p_rows;
+    a.p_sum = b.p_sum;
+    a.p_min = b.p_min;
+    a.p_max = b.p_max;
+    a.r_keys = b.r_keys;
+    a.r_vals = b.r_vals;
+    a.r_count = b.r_count;
+    kern<<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+}  // namespace
+
+bool twophase_supported(int32_t kw, int32_t value_type) {
+    (void)value_type;
+    return kw == 1;
+}
+
+void twophase_free(TwoPhaseBufs& b) {
+    void* ptrs[] = {b.heavy_keys, b.heavy_meta, b.sample_keys, b.sample_cnt, b.hist, b.cand,
+                    b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_keys, b.r_vals, b.r_count};
+    for (void* p : ptrs)
+        if (p) pool_free(p);
+    TwoPhaseBufs fresh;
+    b = fresh;
+}
+
+int twophase_prepare(TwoPhaseBufs& b, const GView& g, int32_t parts, uint32_t fields,
+                     int32_t value_type, bool has_values, int64_t n, hipStream_t s) {
+    const DeviceInfo& d = device_info();
+    const int vbytes = has_values ? type_size(value_type) : 0;
+    if (b.r_keys && b.rows_alloc >= n && b.vbytes == vbytes && b.P == parts) return QK_SUCCESS;
+    const bool had_heavy = b.heavy_ready;
+    twophase_free(b);
+    b.W = d.cus > 0 ? d.cus : 256;
+    b.P = parts;
+    b.vbytes = vbytes;
+    b.block = kBlock;
+    const size_t lds = std::min<size_t>(d.lds_per_block, 160 * 1024);
+    // phase-1 LDS: at most half of the hash slots hold heavy keys
+    int32_t H1 = 64;
+    while (p1_lds_bytes(H1 * 2, parts, fields) <= lds) H1 *= 2;
+    int32_t lo = H1, hi = H1 * 2;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) / 2;
+        if (p1_lds_bytes(mid, parts, fields) <= lds) lo = mid; else hi = mid;
+    }
+    b.H1 = lo;
+    b.kmax = std::min<int32_t>(b.H1 / 2, 4096);
+    int32_t H2 = 64;
+    while (p2_lds_bytes(H2 * 2, b.W, fields) <= lds) H2 *= 2;
+    lo = H2;
+    hi = H2 * 2;
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) / 2;
+        if (p2_lds_bytes(mid, b.W, fields) <= lds) lo = mid; else hi = mid;
+    }
+    b.H2 = lo;
+    // the heavy keys of one partition must fit the phase-2 table below its 4/5 fill limit
+    b.kmax = std::min<int32_t>(b.kmax, b.H2 * 3 / 5);
+    // regions: twice the mean rows per (workgroup, partition), at least 64
+    const int64_t mean = (n + (int64_t)b.W * parts - 1) / ((int64_t)b.W * parts);
+    b.cap = (int32_t)std::max<int64_t>(64, ((2 * mean + 32) + 31) / 32 * 32);
+    b.rows_alloc = n;
+    const int64_t sslots = (int64_t)1 << (b.sample_bits + 1);
+    const int64_t kp = b.kmax + 2;
+    const int64_t regions = (int64_t)b.W * parts * b.cap;
+    int st = QK_SUCCESS;
+    auto A = [&](void** ptr, size_t bytes) {
+        if (st == QK_SUCCESS) st = pool_alloc(ptr, bytes);
+    };
+    A((void**)&b.heavy_keys, (size_t)b.kmax * 8);
+    A((void**)&b.heavy_meta, (size_t)(parts + 8) * 4);
+    A((void**)&b.sample_keys, (size_t)sslots * 8);
+    A((void**)&b.sample_cnt, (size_t)sslots * 4);
+    A((void**)&b.hist, (size_t)kHistBins * 4);
+    A((void**)&b.cand, (size_t)b.kmax * 8);
+    A((void**)&b.p_cnt, (size_t)b.W * kp * 4);
+    A((void**)&b.p_rows, (size_t)b.W * kp * 4);
+    if (fields & F_SUM) A((void**)&b.p_sum, (size_t)b.W * kp * 8);
+    if (fields & F_MIN) A((void**)&b.p_min, (size_t)b.W * kp * 8);
+    if (fields & F_MAX) A((void**)&b.p_max, (size_t)b.W * kp * 8);
+    A((void**)&b.r_keys, (size_t)regions * 8);
+    if (vbytes) A((void**)&b.r_vals, (size_t)regions * vbytes);
+    A((void**)&b.r_count, (size_t)b.W * parts * 4);
+    if (st != QK_SUCCESS) {
+        twophase_free(b);
+        return st;
+    }
+    // no heavy set until the next sample: K = 0, every partition empty
+    QK_HIP(hipMemsetAsync(b.heavy_meta, 0, (size_t)(parts + 8) * 4, s));
+    b.heavy_ready = false;
+    (void)had_heavy;
+    return QK_SUCCESS;
+}
+
+int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                    int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
+                    const PredSet& ps, uint32_t vpred, int64_t n, int mode, hipStream_t s) {
+    const int64_t sslots = (int64_t)1 << (b.sample_bits + 1);
+    const int64_t nsamples = std::min<int64_t>(n, (int64_t)1 << b.sample_bits);
+    const int64_t stride = std::max<int64_t>(1, n / nsamples);
+    const int cus = device_info().cus;
+    k_tp_clear_sample<<<grid_cap(sslots, 256, cus * 8), 256, 0, s>>>(b.sample_keys, b.sample_cnt,
+                                                                     (uint64_t)sslots, b.hist, b.heavy_meta);
+    QK_HIP(hipGetLastError());
+    int st;
+    if (!vals) {
+        st = launch_sample_m<NoValue>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, nsamples, stride, sslots - 1, s);
+    } else {
+        switch (vtype) {
+        case QK_INT: st = launch_sample_m<int32_t>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, nsamples, stride, sslots - 1, s); break;
+        case QK_LONG: st = launch_sample_m<int64_t>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, nsamples, stride, sslots - 1, s); break;
+        case QK_FLOAT: st = launch_sample_m<float>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, nsamples, stride, sslots - 1, s); break;
+        default: st = launch_sample_m<double>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, nsamples, stride, sslots - 1, s); break;
+        }
+    }
+    if (st != QK_SUCCESS) return st;
+    k_tp_hist<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots, b.hist);
+    k_tp_threshold<<<1, 64, 0, s>>>(b.hist, b.kmax, b.heavy_meta);
+    k_tp_collect<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots,
+                                                               b.kmax, b.heavy_meta, b.cand);
+    k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.kmax, b.P, g.part_shift, b.heavy_meta, b.heavy_keys);
+    QK_HIP(hipGetLastError());
+    b.heavy_ready = true;
+    return QK_SUCCESS;
+}
+
+int twophase_block(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                   int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
+                   const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, int mode,
+                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0) QK_HIP(hipEventRecord(ev0, s));
+    int st;
+    if (!vals) {
+        st = launch_p1_m<NoValue>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    } else {
+        switch (vtype) {
+        case QK_INT: st = launch_p1_m<int32_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_LONG: st = launch_p1_m<int64_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_FLOAT: st = launch_p1_m<float>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        default: st = launch_p1_m<double>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        }
+    }
+    if (st != QK_SUCCESS) return st;
+    if (ev1) QK_HIP(hipEventRecord(ev1, s));
+    if (!vals) return launch_p2<NoValue>(b, g, fields, s);
+    switch (vtype) {
+    case QK_INT: return launch_p2<int32_t>(b, g, fields, s);
+    case QK_LONG: return launch_p2<int64_t>(b, g, fields, s);
+    case QK_FLOAT: return launch_p2<float>(b, g, fields, s);
+    default: return launch_p2<double>(b, g, fields, s);
+    }
+}
+
+}  // namespace qk

@@ -1,0 +1,69 @@
+// twophase.hpp - host interface of the two-phase partitioned GROUP BY (twophase.hip).
+//
+// Phase 1 streams a block once: rows passing the WHERE whose key is one of the table's heavy
+// hitters (sampled from the first block) are aggregated into the workgroup's LDS table and
+// written once per workgroup as a partial; every other row is appended, without atomics, to
+// the workgroup's private region of its hash partition.  Phase 2 gives each partition to one
+// workgroup, which folds the partition's rows and the heavy partials in LDS and merges them
+// into the partition's sub-table of the global table, which no other workgroup touches, so
+// the merge needs no atomics.  Rows that do not fit (NULL values, full regions) take the
+// atomic path in phase 1.
+#pragma once
+
+#include "groupby.hpp"
+
+namespace qk {
+
+struct TwoPhaseBufs {
+    int32_t W = 0;          // phase-1 workgroups
+    int32_t P = 1;          // partitions (= sub-tables of the global table)
+    int32_t cap = 0;        // rows per (workgroup, partition) region
+    int32_t vbytes = 0;     // bytes per value in the regions
+    int32_t kmax = 0;       // heavy keys at most
+    int32_t H1 = 0;         // phase-1 LDS hash slots
+    int32_t H2 = 0;         // phase-2 LDS hash slots
+    int32_t block = 1024;   // threads per workgroup (both phases)
+    int32_t sample_bits = 18;
+    int64_t rows_alloc = 0; // block rows the regions were sized for
+    // heavy set
+    uint64_t* heavy_keys = nullptr;   // [kmax], sorted by partition
+    int32_t* heavy_meta = nullptr;    // [0] = K, [1] = threshold, [2..] = start per partition (P+1)
+    uint64_t* sample_keys = nullptr;  // [2^(sample_bits+1)]
+    uint32_t* sample_cnt = nullptr;
+    uint32_t* hist = nullptr;         // [kHistBins]
+    uint64_t* cand = nullptr;         // [kmax] unsorted heavy candidates
+    // per phase-1 workgroup partials [W][kmax + 2]
+    uint32_t* p_cnt = nullptr;
+    uint32_t* p_rows = nullptr;
+    uint64_t* p_sum = nullptr;
+    uint64_t* p_min = nullptr;
+    uint64_t* p_max = nullptr;
+    // cold regions [W][P][cap]
+    uint64_t* r_keys = nullptr;
+    void* r_vals = nullptr;
+    uint32_t* r_count = nullptr;      // [W][P]
+    bool heavy_ready = false;
+};
+
+constexpr int kHistBins = 1024;
+
+// Whether the two-phase path applies: single-word keys and a value type it handles.
+bool twophase_supported(int32_t kw, int32_t value_type);
+
+// (Re)size the buffers for blocks of n rows; fields/value type decide the LDS layouts.
+int twophase_prepare(TwoPhaseBufs& b, const GView& g, int32_t parts, uint32_t fields,
+                     int32_t value_type, bool has_values, int64_t n, hipStream_t s);
+void twophase_free(TwoPhaseBufs& b);
+
+// Sample the block and build the heavy-hitter set (device-side, no host round trip).
+int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                    int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
+                    const PredSet& ps, uint32_t vpred, int64_t n, int mode, hipStream_t s);
+
+// Phase 1 + phase 2 over one block.
+int twophase_block(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                   int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
+                   const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, int mode,
+                   hipStream_t s, hipEvent_t ev_p1_start, hipEvent_t ev_p1_end);
+
+}  // namespace qk

@@ -250,6 +250,10 @@ class GPUGroupBy:
     def set_tuning(self, block: int = -1, lds_bytes: int = -1, wgs_per_cu: int = -1) -> None:
         call("qk_gb_set_tuning", self.h, block, lds_bytes, wgs_per_cu)
 
+    def set_path(self, path: int = -1, min_rows: int = -1) -> None:
+        """-1 automatic, 0 one-pass kernel, 1 two-phase partitioned kernels."""
+        call("qk_gb_set_path", self.h, path, min_rows)
+
     def set_profiling(self, on: bool = True) -> None:
         call("qk_gb_set_profiling", self.h, 1 if on else 0)
 

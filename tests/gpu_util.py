@@ -36,7 +36,7 @@ class Where:
 
 def gpu_group_by(aggs, keys, value=None, key_nulls=None, value_nulls=None, where=(), mask=None,
                  block=None, capacity=1 << 16, flags=0, tuning=None, out_types=None,
-                 extra_cols=None, value_type=None):
+                 extra_cols=None, value_type=None, path=None):
     """keys: list of numpy arrays; key_nulls / value_nulls: bool arrays or None.
     Rows are processed in blocks of `block` rows (each block uploaded with its own masks)."""
     keys = [np.ascontiguousarray(k) for k in keys]
@@ -47,6 +47,8 @@ def gpu_group_by(aggs, keys, value=None, key_nulls=None, value_nulls=None, where
     gb = qk.GPUGroupBy(aggs, ktypes, vtype, capacity, 1, out_types=out_types, flags=flags)
     if tuning:
         gb.set_tuning(**tuning)
+    if path is not None:
+        gb.set_path(path, 0)
     block = block or n
     extra_cols = extra_cols or {}
     for s in range(0, n, block):

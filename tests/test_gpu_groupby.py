@@ -253,3 +253,77 @@ def test_workload_composite_bit_exact():
                        flags=qk.GB_KEYS_NOT_NULL, extra_cols=extra)
     cpu = cpu_group_by(aggs, [d["a"], d["b"]], d["value"], where=w, extra_cols=extra)
     compare(gpu, cpu, 4)
+
+
+# ---- both row paths: two-phase partitioned and one-pass ------------------------------------
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("kt,vt", [(np.int64, np.float32), (np.int32, np.int32), (np.float64, np.int64),
+                                   (np.int64, np.float64), (np.float32, np.float32)])
+def test_paths_agree_with_oracle(path, kt, vt):
+    rng = np.random.default_rng(np.dtype(kt).itemsize * 10 + np.dtype(vt).itemsize + path)
+    n = 700_001
+    if np.issubdtype(kt, np.integer):
+        keys = datagen.zipf_keys(5, 1.1, 40_000, n).astype(kt)
+    else:
+        keys = (datagen.zipf_keys(5, 1.1, 40_000, n) % 100_000).astype(kt) * 0.5
+    vals = make_vals(vt, n, rng)
+    kn = [rng.random(n) < 0.01]
+    vn = rng.random(n) < 0.05
+    aggs = [qk.AGG_SUM, qk.AGG_COUNT, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_AVG, qk.AGG_COUNT_ALL]
+    real = (0, 4) if np.issubdtype(vt, np.floating) else ()
+    gpu = gpu_group_by(aggs, [keys], vals, kn, vn, block=1 << 18, capacity=100_000, path=path)
+    cpu = cpu_group_by(aggs, [keys], vals, kn, vn)
+    compare(gpu, cpu, len(aggs), real)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+def test_paths_where_mask_and_count_only(path):
+    rng = np.random.default_rng(31 + path)
+    n = 1_500_000
+    keys = datagen.zipf_keys(9, 1.1, 200_000, n)
+    vals = rng.lognormal(0, 1, n).astype(np.float32)
+    vn = rng.random(n) < 0.05
+    w = [Where(qk.GREATER_EQUAL, "value", 0.5, qk.FLOAT), Where(qk.LESS_EQUAL, "value", 4.0, qk.FLOAT)]
+    aggs = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT]
+    compare(gpu_group_by(aggs, [keys], vals, None, vn, where=w, capacity=300_000, path=path,
+                         flags=qk.GB_KEYS_NOT_NULL),
+            cpu_group_by(aggs, [keys], vals, None, vn, where=w), 3, real_aggs=(0, 1))
+    mask = (rng.random(n) < 0.3).astype(np.int8)
+    compare(gpu_group_by([qk.AGG_COUNT, qk.AGG_COUNT_ALL], [keys], None, None, None, mask=mask,
+                         capacity=300_000, path=path, value_type=qk.INT),
+            cpu_group_by([qk.AGG_COUNT, qk.AGG_COUNT_ALL], [keys], None, None, None, mask=mask,
+                         value_type=qk.INT), 2)
+
+
+def test_two_phase_uniform_high_cardinality_and_sentinel():
+    # no heavy hitters: every row goes through the partitions; the empty-marker key and NULL
+    # keys go through the side slots
+    rng = np.random.default_rng(77)
+    n = 2_000_000
+    keys = rng.integers(-(2**62), 2**62, n // 4).astype(np.int64)[rng.integers(0, n // 4, n)]
+    keys[::1000] = np.iinfo(np.int64).min
+    kn = [rng.random(n) < 0.001]
+    vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    gpu = gpu_group_by(aggs, [keys], vals, kn, None, block=1 << 20, capacity=n, path=1)
+    cpu = cpu_group_by(aggs, [keys], vals, kn, None)
+    compare(gpu, cpu, 4)
+
+
+def test_two_phase_small_table_all_heavy_over_blocks():
+    d = datagen.small(1 << 21, seed=4)
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    w = [Where(qk.GREATER, "value", 0, qk.INT)]
+    gpu = gpu_group_by(aggs, [d["key"]], d["value"], where=w, block=1 << 19, capacity=1000,
+                       out_types=[qk.LONG, 0, 0, 0], path=1)
+    cpu = cpu_group_by(aggs, [d["key"]], d["value"], where=w, out_types=[qk.LONG, 0, 0, 0])
+    compare(gpu, cpu, 4)
+
+
+def test_two_phase_table_full_reported():
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 50_000, 2_000_000).astype(np.int32)
+    with pytest.raises(qk.QueryEngineError) as e:
+        gpu_group_by(qk.AGG_COUNT, [keys], None, capacity=10_000, path=1, value_type=qk.INT)
+    assert e.value.code == qk.QK_HASH_TABLE_FULL
