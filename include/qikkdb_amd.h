@@ -284,6 +284,21 @@ int qk_gb_set_tuning(qk_groupby h, int32_t block, int32_t lds_bytes, int32_t wgs
  * threshold unchanged.  Results do not depend on the path. */
 int qk_gb_set_path(qk_groupby h, int32_t path, int64_t min_rows);
 
+/* Deferred merging on the two-phase path (no reference counterpart).  A table created with
+ * QK_GB_ASYNC may keep the partitioned rows of several blocks and merge them into the table
+ * once, when the table is next read (qk_gb_get_results, qk_gb_group_count,
+ * qk_gb_check_error) or when the buffers are full.  rows: how many rows may be kept
+ * (-1 = default: four blocks, at most 2^28 rows; 0 = merge after every block).  The
+ * partition buffers take about 24 bytes per kept row. */
+int qk_gb_set_defer_rows(qk_groupby h, int64_t rows);
+
+/* Internal state for tuning (no reference counterpart); waits for the table's work.
+ * out[0] path of the last block (0 one-pass, 1 two-phase), out[1] heavy keys, out[2] heavy
+ * count threshold, out[3] rows written to partition regions by the last block, out[4]
+ * phase-1 workgroups, out[5] partitions, out[6] rows per region, out[7] phase-1 LDS slots,
+ * out[8] phase-2 LDS slots, out[9] heavy key limit.  n = entries to fill (at most 10). */
+int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

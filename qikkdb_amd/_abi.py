@@ -111,6 +111,8 @@ PROTOTYPES = {
     "qk_gb_describe": [_vp, _P(_i32), _P(_i64), _P(_i32)],
     "qk_gb_set_tuning": [_vp, _i32, _i32, _i32],
     "qk_gb_set_path": [_vp, _i32, _i64],
+    "qk_gb_set_defer_rows": [_vp, _i64],
+    "qk_gb_stats": [_vp, _P(_i64), _i32, _vp],
 }
 
 _lib = None

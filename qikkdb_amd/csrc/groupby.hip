@@ -507,6 +507,7 @@ struct qk_groupby_s {
     int32_t lds_budget = -1;  // bytes of LDS per workgroup; -1 = half of a CU's LDS, 0 = none
     int32_t wgs_per_cu = 0;   // 0 = as many as fit
     int32_t last_H = 0;
+    int32_t last_path = 0;
     // profiling
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -556,7 +557,14 @@ int init_table(qk_groupby_s* h, hipStream_t s) {
     QK_HIP(hipGetLastError());
     QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
     h->tp.heavy_ready = false;
+    h->tp.pending_rows = 0;  // rows of the previous query still in the regions are dropped
     return QK_SUCCESS;
+}
+
+// Merge rows whose phase 2 was deferred; everything that reads the table calls this first.
+int flush_pending(qk_groupby_s* h, hipStream_t s) {
+    if (h->tp.pending_rows == 0) return QK_SUCCESS;
+    return twophase_flush(h->tp, view_of(h), h->fields, s);
 }
 
 int read_counters(qk_groupby_s* h, hipStream_t s, TableCounters* out) {
@@ -639,15 +647,26 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
                      (h->twophase_mode == 1 || (h->twophase_mode < 0 && n >= h->twophase_min_rows));
     int st;
     if (two) {
-        QK_TRY(twophase_prepare(h->tp, g, h->parts, h->fields, h->value_type, values != nullptr, n, s));
-        if (!h->tp.heavy_ready)
-            QK_TRY(twophase_sample(h->tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n, mode, s));
-        h->last_H = h->tp.H1;
-        st = twophase_block(h->tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n,
-                            h->fields, mode, s, ev ? ev->first : nullptr, ev ? ev->second : nullptr);
+        // Phase 2 is deferred across blocks only for asynchronous tables: a synchronous table
+        // reports a full table at the block that filled it, as the reference does.
+        const bool defer = (h->flags & QK_GB_ASYNC) != 0 && h->tp.defer_rows != 0;
+        TwoPhaseBufs& tp = h->tp;
+        if (tp.pending_rows > 0 &&
+            (twophase_needs_realloc(tp, h->parts, h->value_type, values != nullptr, n, defer) ||
+             tp.pending_rows + n > tp.rows_alloc || tp.pend_vtype != vtype))
+            QK_TRY(twophase_flush(tp, g, h->fields, s));
+        QK_TRY(twophase_prepare(tp, h->parts, h->fields, h->value_type, values != nullptr, n, defer, s));
+        if (!tp.heavy_ready)
+            QK_TRY(twophase_sample(tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n, mode, s));
+        h->last_H = tp.H1;
+        h->last_path = 1;
+        st = twophase_phase1(tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n,
+                             h->fields, mode, s, ev ? ev->first : nullptr, ev ? ev->second : nullptr);
+        if (st == QK_SUCCESS && !defer) st = twophase_flush(tp, g, h->fields, s);
     } else {
         const int32_t H = choose_lds_slots(h);
         h->last_H = H;
+        h->last_path = 0;
         if (ev) QK_HIP(hipEventRecord(ev->first, s));
         switch (h->ks.kw) {
         case 1: st = launch_rows_v<1>(vtype, mode, g, h->ks, kc, values, value_nulls, mask, ps, vpred, n, h->fields, H, h->block, h->wgs_per_cu, s); break;
@@ -866,12 +885,14 @@ int qk_gb_process_block_where(qk_groupby h, const void* const* keys,
 
 int qk_gb_check_error(qk_groupby h, void* stream) {
     if (!h) return fail(QK_ERROR, "qk_gb_check_error: null table");
+    QK_TRY(flush_pending(h, as_stream(stream)));
     return take_error(h, as_stream(stream));
 }
 
 int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream) {
     if (!h || !out) return fail(QK_ERROR, "qk_gb_group_count: null argument");
     hipStream_t s = as_stream(stream);
+    QK_TRY(flush_pending(h, s));
     TableCounters c;
     QK_TRY(read_counters(h, s, &c));
     uint64_t side = 0;
@@ -903,6 +924,7 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
         if (out_values) out_values[a] = nullptr;
         if (out_value_nulls) out_value_nulls[a] = nullptr;
     }
+    QK_TRY(flush_pending(h, s));
     QK_TRY(take_error(h, s));
     int64_t groups = 0;
     QK_TRY(qk_gb_group_count(h, &groups, stream));
@@ -1032,11 +1054,38 @@ int qk_gb_set_tuning(qk_groupby h, int32_t block, int32_t lds_bytes, int32_t wgs
     return QK_SUCCESS;
 }
 
+int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
+    if (!h || !out || n < 0) return fail(QK_ERROR, "qk_gb_stats: bad argument");
+    hipStream_t s = as_stream(stream);
+    QK_HIP(hipStreamSynchronize(s));
+    int64_t v[10] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax};
+    if (h->tp.heavy_meta) {
+        int32_t meta[2];
+        QK_HIP(hipMemcpy(meta, h->tp.heavy_meta, 8, hipMemcpyDeviceToHost));
+        v[1] = meta[0];
+        v[2] = meta[1];
+    }
+    if (h->tp.r_count) {
+        std::vector<uint32_t> rc((size_t)h->tp.W * h->tp.P);
+        QK_HIP(hipMemcpy(rc.data(), h->tp.r_count, rc.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t c : rc) v[3] += c;
+    }
+    for (int i = 0; i < n && i < 10; i++) out[i] = v[i];
+    return QK_SUCCESS;
+}
+
 int qk_gb_set_path(qk_groupby h, int32_t path, int64_t min_rows) {
     if (!h) return fail(QK_ERROR, "qk_gb_set_path: null table");
     if (path < -1 || path > 1) return fail(QK_ERROR, "qk_gb_set_path: path must be -1, 0 or 1");
     h->twophase_mode = path;
     if (min_rows >= 0) h->twophase_min_rows = min_rows;
+    return QK_SUCCESS;
+}
+
+int qk_gb_set_defer_rows(qk_groupby h, int64_t rows) {
+    if (!h) return fail(QK_ERROR, "qk_gb_set_defer_rows: null table");
+    if (rows < -1) return fail(QK_ERROR, "qk_gb_set_defer_rows: rows must be -1, 0 or positive");
+    h->tp.defer_rows = rows;
     return QK_SUCCESS;
 }
 

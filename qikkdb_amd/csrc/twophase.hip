@@ -1,8 +1,9 @@
 // twophase.hip - two-phase partitioned GROUP BY for single-word keys (see twophase.hpp).
 //
-// Heavy hitters: a strided sample of the first block is counted in a device hash table; the
-// keys seen at least `t` times (t chosen from the count histogram so that at most kmax keys
-// qualify) become the heavy set, sorted by partition.  All of it runs on the device.
+// Heavy hitters: a strided sample of the first block is counted (per workgroup in LDS, then
+// merged into a device hash table); the keys seen at least `t` times (t chosen from the count
+// histogram so that at most kmax keys qualify) become the heavy set, sorted by partition.
+// All of it runs on the device.
 #include <algorithm>
 
 #include "twophase.hpp"
@@ -12,7 +13,10 @@ namespace qk {
 namespace {
 
 constexpr int kP1Steps = 4;   // rows per thread per tile in phase 1
+constexpr int kP2Rows = 4;    // cold rows per thread per chunk in phase 2
 constexpr int kBlock = 1024;
+constexpr int kBucket = 4;    // slots per bucket of the phase-1 heavy table
+constexpr int kSampleLds = 2048;
 
 // VV is VT, or a placeholder type when the table reads no value column.
 template <typename VT, int MODE, typename VV>
@@ -37,6 +41,23 @@ __device__ __forceinline__ bool row_passes(const PredSet& preds, uint32_t vpred,
     return true;
 }
 
+// Exclusive prefix sum over the threads of a workgroup (blockDim a multiple of 64, <= 1024);
+// *total receives the sum.  `wsum` is LDS for 16 values.  Every thread must call it.
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* wsum, int32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int32_t incl = wave_incl_scan(v, lane);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int32_t before = 0, all = 0;
+    for (int w = 0; w < nw; w++) {
+        if (w < wid) before += wsum[w];
+        all += wsum[w];
+    }
+    *total = all;
+    __syncthreads();
+    return before + incl - v;
+}
+
 // ---- sampling --------------------------------------------------------------------------
 
 __global__ void k_tp_clear_sample(uint64_t* keys, uint32_t* cnt, uint64_t slots, uint32_t* hist,
@@ -52,6 +73,27 @@ __global__ void k_tp_clear_sample(uint64_t* keys, uint32_t* cnt, uint64_t slots,
     }
 }
 
+// Add `c` occurrences of key k to the device sample table.
+__device__ __forceinline__ void sample_add(uint64_t* skeys, uint32_t* scnt, uint64_t smask,
+                                           uint64_t k, uint32_t c) {
+    uint64_t s = hash64(k) & smask;
+    for (uint64_t p = 0; p <= smask; p++) {
+        uint64_t cur = *(volatile uint64_t*)(skeys + s);
+        if (cur == kEmpty) {
+            cur = atomicCAS((unsigned long long*)(skeys + s), (unsigned long long)kEmpty,
+                            (unsigned long long)k);
+            if (cur == kEmpty) cur = k;
+        }
+        if (cur == k) {
+            atomicAdd(scnt + s, c);
+            return;
+        }
+        s = (s + 1) & smask;
+    }
+}
+
+// Each workgroup counts a contiguous range of samples in LDS first, so that a hot key costs
+// one device atomic per workgroup instead of one per sample.
 template <typename VT, int MODE>
 __global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const VT* __restrict__ vals,
                                                    const uint64_t* __restrict__ vnull,
@@ -59,8 +101,16 @@ __global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const
                                                    uint32_t vpred, int64_t n, int64_t stride,
                                                    int64_t nsamples, uint64_t* skeys,
                                                    uint32_t* scnt, uint64_t smask) {
-    const int64_t gs = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsamples; i += gs) {
+    __shared__ uint64_t lk[kSampleLds];
+    __shared__ uint32_t lc[kSampleLds];
+    for (int s = threadIdx.x; s < kSampleLds; s += blockDim.x) {
+        lk[s] = kEmpty;
+        lc[s] = 0;
+    }
+    __syncthreads();
+    const int64_t per = (nsamples + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min(nsamples, i0 + per);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const int64_t r = i * stride;
         if (r >= n) break;
         typename std::conditional<VTraits<VT>::has, VT, int>::type v{};
@@ -71,21 +121,22 @@ __global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const
         bool kn;
         load_key<1>(ks, kc, r, k, kn);
         if (kn || k[0] == kEmpty) continue;  // those keys have side slots anyway
-        uint64_t s = hash64(k[0]) & smask;
-        for (uint64_t p = 0; p <= smask; p++) {
-            uint64_t cur = *(volatile uint64_t*)(skeys + s);
-            if (cur == kEmpty) {
-                cur = atomicCAS((unsigned long long*)(skeys + s), (unsigned long long)kEmpty,
-                                (unsigned long long)k[0]);
-                if (cur == kEmpty) cur = k[0];
+        uint32_t s = (uint32_t)(hash64(k[0]) >> 53) & (kSampleLds - 1);
+        bool done = false;
+        for (int p = 0; p < 16 && !done; p++) {
+            unsigned long long old = atomicCAS((unsigned long long*)&lk[s], (unsigned long long)kEmpty,
+                                               (unsigned long long)k[0]);
+            if (old == kEmpty || old == k[0]) {
+                atomicAdd(&lc[s], 1u);
+                done = true;
             }
-            if (cur == k[0]) {
-                atomicAdd(scnt + s, 1u);
-                break;
-            }
-            s = (s + 1) & smask;
+            s = (s + 1) & (kSampleLds - 1);
         }
+        if (!done) sample_add(skeys, scnt, smask, k[0], 1u);
     }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kSampleLds; s += blockDim.x)
+        if (lc[s]) sample_add(skeys, scnt, smask, lk[s], lc[s]);
 }
 
 __global__ void __launch_bounds__(256) k_tp_hist(const uint64_t* __restrict__ skeys,
@@ -102,18 +153,23 @@ __global__ void __launch_bounds__(256) k_tp_hist(const uint64_t* __restrict__ sk
         if (lh[b]) atomicAdd(hist + b, lh[b]);
 }
 
-// Smallest count t >= 2 such that at most kmax sampled keys were seen t times or more.
-__global__ void k_tp_threshold(const uint32_t* hist, int32_t kmax, int32_t* meta) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t cum = 0;
-    int32_t t = kHistBins;
-    for (int b = kHistBins - 1; b >= 2; b--) {
-        if (cum + hist[b] > (uint32_t)kmax) break;
-        cum += hist[b];
-        t = b;
+// Smallest count t >= 2 such that at most kmax sampled keys were seen t times or more
+// (kHistBins threads: thread i holds bin kHistBins-1-i, so a prefix sum is a suffix sum).
+__global__ void __launch_bounds__(kHistBins) k_tp_threshold(const uint32_t* hist, int32_t kmax,
+                                                            int32_t* meta) {
+    __shared__ int32_t wsum[16];
+    __shared__ int32_t best;
+    const int i = threadIdx.x, b = kHistBins - 1 - i;
+    if (i == 0) best = kHistBins;
+    const int32_t v = (int32_t)min(hist[b], (uint32_t)0x3fffffff);
+    int32_t total;
+    const int32_t suffix = block_excl_scan(v, wsum, &total) + v;  // keys seen >= b times
+    if (b >= 2 && suffix <= kmax) atomicMin(&best, b);
+    __syncthreads();
+    if (i == 0) {
+        meta[1] = best;
+        meta[0] = 0;
     }
-    meta[1] = t;
-    meta[0] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_tp_collect(const uint64_t* __restrict__ skeys,
@@ -135,6 +191,7 @@ __global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ c
                                                   uint64_t* heavy) {
     __shared__ int32_t cnt[1024];
     __shared__ int32_t cur[1024];
+    __shared__ int32_t wsum[16];
     const int t = threadIdx.x;
     const int32_t K = min(meta[0], kmax);
     for (int p = t; p < P; p += blockDim.x) cnt[p] = 0;
@@ -145,14 +202,14 @@ __global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ c
         atomicAdd(&cnt[p], 1);
     }
     __syncthreads();
+    int32_t total;
+    const int32_t start = block_excl_scan(t < P ? cnt[t] : 0, wsum, &total);
+    if (t < P) {
+        cur[t] = start;
+        meta[2 + t] = start;
+    }
     if (t == 0) {
-        int32_t run = 0;
-        for (int p = 0; p < P; p++) {
-            cur[p] = run;
-            meta[2 + p] = run;
-            run += cnt[p];
-        }
-        meta[2 + P] = run;
+        meta[2 + P] = total;
         meta[0] = K;
     }
     __syncthreads();
@@ -177,34 +234,64 @@ struct P1Args {
     uint64_t* r_keys;
     void* r_vals;
     uint32_t* r_count;
+    const uint64_t* zero;  // a zero word (stands in for an absent null bitmap)
+    int32_t accumulate;    // regions and partials hold rows of earlier blocks
 };
 
-__host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, uint32_t fields) {
+// H hashed slots in H/kBucket buckets, + 2 side slots for the partial arrays.
+__host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, int32_t kmax, uint32_t fields) {
     const size_t T = (size_t)H + 2;
     size_t b = (size_t)H * 8;                                  // keys
     b += T * 8 * (((fields & F_SUM) ? 1 : 0) + ((fields & F_MIN) ? 1 : 0) + ((fields & F_MAX) ? 1 : 0));
     b += T * 4 * 2;                                            // cnt, rows
-    b += (size_t)H * 4;                                        // heavy index
+    b += (size_t)kmax * 4;                                     // slot of each heavy key
     b += (size_t)P * 4 + 16;                                   // cursors, counters
     return b;
 }
 
-// LDS home slot of a key in a table of H slots (high hash bits, range-reduced).
-__device__ __forceinline__ uint32_t lds_home(uint64_t h, int32_t H) {
-    return (uint32_t)(((h >> 32) * (uint64_t)(uint32_t)H) >> 32);
+// The two candidate buckets of a key (independent halves of its hash).
+__device__ __forceinline__ void buckets_of(uint64_t h, uint32_t nb, uint32_t& b1, uint32_t& b2) {
+    b1 = (uint32_t)(((h >> 32) * (uint64_t)nb) >> 32);
+    b2 = (uint32_t)(((h & 0xffffffffull) * (uint64_t)nb) >> 32);
 }
 
-template <typename VT, int MODE>
+// Slot of key k in the heavy table, or -1 (two buckets of kBucket keys, read as 128-bit words).
+__device__ __forceinline__ int heavy_find(const uint64_t* lkey, uint32_t nb, uint64_t k, uint64_t h) {
+    uint32_t b1, b2;
+    buckets_of(h, nb, b1, b2);
+    const ulonglong2* p1 = reinterpret_cast<const ulonglong2*>(lkey + (size_t)b1 * kBucket);
+    const ulonglong2* p2 = reinterpret_cast<const ulonglong2*>(lkey + (size_t)b2 * kBucket);
+    const ulonglong2 a0 = p1[0], a1 = p1[1], c0 = p2[0], c1 = p2[1];
+    int s = -1;
+    if (a0.x == k) s = (int)(b1 * kBucket + 0);
+    if (a0.y == k) s = (int)(b1 * kBucket + 1);
+    if (a1.x == k) s = (int)(b1 * kBucket + 2);
+    if (a1.y == k) s = (int)(b1 * kBucket + 3);
+    if (c0.x == k) s = (int)(b2 * kBucket + 0);
+    if (c0.y == k) s = (int)(b2 * kBucket + 1);
+    if (c1.x == k) s = (int)(b2 * kBucket + 2);
+    if (c1.y == k) s = (int)(b2 * kBucket + 3);
+    return s;
+}
+
+// KT is the element type of a single integer key column, or void for any key layout (read
+// through the key spec's runtime switch).  With a typed column, the loads of a whole tile are
+// issued before any of them is used.
+template <typename KT>
+constexpr bool kTypedKey = !std::is_same<KT, void>::value;
+
+template <typename VT, int MODE, typename KT>
 __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCols kc,
                                                       const VT* __restrict__ vals,
                                                       const uint64_t* __restrict__ vnull,
                                                       const int8_t* __restrict__ mask, PredSet preds,
                                                       uint32_t vpred, int64_t n, uint32_t fields,
                                                       P1Args a) {
-    extern __shared__ uint64_t lds[];
+    extern __shared__ __align__(16) uint64_t lds[];
     constexpr bool HAS_V = VTraits<VT>::has;
     constexpr bool REAL = VTraits<VT>::real;
     const int H = a.H, T = H + 2, bd = blockDim.x, tid = threadIdx.x;
+    const uint32_t nb = (uint32_t)(H / kBucket);
     uint64_t* lkey = lds;
     uint64_t* p = lkey + H;
     uint64_t* lsum = (fields & F_SUM) ? p : nullptr;
@@ -215,88 +302,104 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
     p += (fields & F_MAX) ? T : 0;
     uint32_t* lcnt = (uint32_t*)p;
     uint32_t* lrows = lcnt + T;
-    int32_t* lidx = (int32_t*)(lrows + T);
-    uint32_t* cursor = (uint32_t*)(lidx + H);
+    int32_t* slot_of = (int32_t*)(lrows + T);  // [kmax]
+    uint32_t* cursor = (uint32_t*)(slot_of + a.kmax);
     uint32_t* nnew = cursor + a.P;
 
     for (int s = tid; s < T; s += bd) {
-        if (s < H) {
-            lkey[s] = kEmpty;
-            lidx[s] = -1;
-        }
+        if (s < H) lkey[s] = kEmpty;
         lcnt[s] = 0;
         lrows[s] = 0;
         if (lsum) lsum[s] = 0;
         if (lmin) lmin[s] = ~0ull;
         if (lmax) lmax[s] = 0;
     }
-    for (int q = tid; q < a.P; q += bd) cursor[q] = 0;
+    const int64_t region0 = (int64_t)blockIdx.x * a.P;
+    // regions continue where the previous block left them until phase 2 consumes them
+    for (int q = tid; q < a.P; q += bd) cursor[q] = a.accumulate ? a.r_count[region0 + q] : 0u;
     if (tid == 0) nnew[0] = 0;
     __syncthreads();
+    // place each heavy key in the emptier of its two buckets; a key that finds both full
+    // stays cold (it is then aggregated through its partition like any other key)
     const int32_t K = min(a.meta[0], a.kmax);
     for (int k = tid; k < K; k += bd) {
         const uint64_t key = a.heavy[k];
-        uint32_t s = lds_home(hash64(key), H);
-        for (int it = 0; it < H; it++) {
-            unsigned long long old = atomicCAS((unsigned long long*)&lkey[s], (unsigned long long)kEmpty,
-                                               (unsigned long long)key);
-            if (old == kEmpty) {
-                lidx[s] = k;
-                break;
+        uint32_t b1, b2;
+        buckets_of(hash64(key), nb, b1, b2);
+        int placed = -1;
+        for (int attempt = 0; attempt < 2 * kBucket && placed < 0; attempt++) {
+            int f1 = 0, f2 = 0;
+            for (int j = 0; j < kBucket; j++) {
+                f1 += lkey[b1 * kBucket + j] != kEmpty;
+                f2 += lkey[b2 * kBucket + j] != kEmpty;
             }
-            if (old == key) break;
-            s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
+            const uint32_t bb = (f1 <= f2) ? b1 : b2;
+            if (((f1 <= f2) ? f1 : f2) >= kBucket) break;
+            for (int j = 0; j < kBucket && placed < 0; j++) {
+                unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
+                                                   (unsigned long long)kEmpty, (unsigned long long)key);
+                if (old == kEmpty) placed = (int)(bb * kBucket + j);
+            }
         }
+        slot_of[k] = placed;
     }
     __syncthreads();
 
     const int64_t tile_rows = (int64_t)bd * kP1Steps;
     const int64_t ntiles = (n + tile_rows - 1) / tile_rows;
-    const int64_t region0 = (int64_t)blockIdx.x * a.P;
+    // rows of one wave in one step are 64 consecutive rows starting at a multiple of 64, so
+    // they share one word of each null bitmap, loaded once for the wave
+    const int lane = tid & 63;
+    const int wave_row = __builtin_amdgcn_readfirstlane(tid & ~63);
+    // loads are unconditional so that a tile's loads are all in flight together: rows past
+    // the end re-read the last row (and are dropped below), and a missing bitmap reads a
+    // zero word
+    const uint64_t* vbits = vnull ? vnull : a.zero;
+    const int64_t vstep = vnull ? 1 : 0;
+    const uint64_t* kbits = kc.nm[0] ? kc.nm[0] : a.zero;
+    const int64_t kstep = kc.nm[0] ? 1 : 0;
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        uint64_t key[kP1Steps][1];
-        bool kn[kP1Steps], act[kP1Steps], vn[kP1Steps];
+        uint64_t key[kP1Steps], vword[kP1Steps], kword[kP1Steps];
+        bool kn[kP1Steps];
         typename std::conditional<HAS_V, VT, int>::type val[kP1Steps];
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
-            const int64_t r = tile * tile_rows + (int64_t)k * bd + tid;
-            act[k] = r < n;
-            vn[k] = false;
+            const int64_t r0 = min(tile * tile_rows + (int64_t)k * bd + wave_row, n - 1);
+            const int64_t r = min(r0 + lane, n - 1);
+            vword[k] = vbits[(r0 >> 6) * vstep];
+            kword[k] = 0;
             kn[k] = false;
-            key[k][0] = 0;
-            val[k] = {};
-            if (act[k]) {
-                load_key<1>(ks, kc, r, key[k], kn[k]);
-                if constexpr (HAS_V) val[k] = vals[r];
-                vn[k] = vnull && null_bit(vnull, r);
+            if constexpr (kTypedKey<KT>) {
+                kword[k] = kbits[(r0 >> 6) * kstep];
+                key[k] = (uint64_t)(int64_t)((const KT*)kc.col[0])[r];
+            } else {
+                uint64_t kk[1];
+                load_key<1>(ks, kc, r, kk, kn[k]);
+                key[k] = kk[0];
             }
+            if constexpr (HAS_V) val[k] = vals[r]; else val[k] = 0;
         }
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
-            if (!act[k]) continue;
             const int64_t r = tile * tile_rows + (int64_t)k * bd + tid;
-            if (!row_passes<VT, MODE>(preds, vpred, mask, r, val[k], vn[k])) continue;
-            const uint64_t kk = key[k][0];
-            const uint64_t h = hash64(kk);
-            int ls = -1;
-            if (kn[k]) {
-                ls = H;
-            } else if (kk == kEmpty) {
-                ls = H + 1;
-            } else {
-                uint32_t s = lds_home(h, H);
-                for (int it = 0; it < H; it++) {  // at most half full: a miss ends at an empty slot
-                    const uint64_t cur = lkey[s];
-                    if (cur == kk) {
-                        ls = (int)s;
-                        break;
-                    }
-                    if (cur == kEmpty) break;
-                    s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
-                }
+            if (r >= n) continue;
+            const bool vn = (vword[k] >> lane) & 1;
+            if constexpr (kTypedKey<KT>) {
+                kn[k] = (kword[k] >> lane) & 1;
+                if (kn[k]) key[k] = 0;
             }
+            if (!row_passes<VT, MODE>(preds, vpred, mask, r, val[k], vn)) continue;
+            const uint64_t kk = key[k];
+            const uint64_t h = hash64(kk);
+            int ls;
+            if (kn[k])
+                ls = H;
+            else if (kk == kEmpty)
+                ls = H + 1;
+            else
+                ls = heavy_find(lkey, nb, kk, h);
             Part pr;
-            if constexpr (HAS_V) pr = row_part<VT>(val[k], vn[k]); else pr = row_part<NoValue>(NoValue{}, vn[k]);
+            if constexpr (HAS_V) pr = row_part<VT>(val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
             if (ls >= 0) {
                 atomicAdd(&lrows[ls], 1u);
                 if (pr.cnt) {
@@ -312,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
                 }
                 continue;
             }
-            if (!vn[k]) {
+            if (!vn) {
                 const uint32_t q = part_of(g, h);
                 const uint32_t c = atomicAdd(&cursor[q], 1u);
                 if (c < (uint32_t)a.cap) {
@@ -332,19 +435,32 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
 
     __syncthreads();
     const int64_t pbase = (int64_t)blockIdx.x * (a.kmax + 2);
-    for (int s = tid; s < T; s += bd) {
-        int32_t k;
-        if (s < H) {
-            if (lkey[s] == kEmpty) continue;
-            k = lidx[s];
-        } else {
-            k = a.kmax + (s - H);
+    // heavy partials, indexed by heavy key (zeros for a key this workgroup could not place),
+    // added to what earlier blocks left when phase 2 has not consumed it yet
+    for (int k = tid; k < K + 2; k += bd) {
+        const int s = k < K ? slot_of[k] : H + (k - K);
+        const int64_t at = pbase + (k < K ? k : a.kmax + (k - K));
+        uint32_t rows = s >= 0 ? lrows[s] : 0u, cnt = s >= 0 ? lcnt[s] : 0u;
+        uint64_t sum = (lsum && s >= 0) ? lsum[s] : 0ull;
+        uint64_t mn = (lmin && s >= 0) ? lmin[s] : ~0ull, mx = (lmax && s >= 0) ? lmax[s] : 0ull;
+        if (a.accumulate) {
+            rows += a.p_rows[at];
+            cnt += a.p_cnt[at];
+            if (lsum) {
+                if (REAL)
+                    sum = (uint64_t)__double_as_longlong(__longlong_as_double((long long)sum) +
+                                                         __longlong_as_double((long long)a.p_sum[at]));
+                else
+                    sum += a.p_sum[at];
+            }
+            if (lmin) mn = min(mn, (uint64_t)a.p_min[at]);
+            if (lmax) mx = max(mx, (uint64_t)a.p_max[at]);
         }
-        a.p_rows[pbase + k] = lrows[s];
-        a.p_cnt[pbase + k] = lcnt[s];
-        if (lsum) a.p_sum[pbase + k] = lsum[s];
-        if (lmin) a.p_min[pbase + k] = lmin[s];
-        if (lmax) a.p_max[pbase + k] = lmax[s];
+        a.p_rows[at] = rows;
+        a.p_cnt[at] = cnt;
+        if (lsum) a.p_sum[at] = sum;
+        if (lmin) a.p_min[at] = mn;
+        if (lmax) a.p_max[at] = mx;
     }
     for (int q = tid; q < a.P; q += bd) a.r_count[region0 + q] = min(cursor[q], (uint32_t)a.cap);
     if (tid == 0) g_count_new(g, nnew[0]);
@@ -381,7 +497,8 @@ __device__ __forceinline__ uint32_t p2_home(uint64_t h, int32_t H2) {
 
 template <typename VT>
 __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, P2Args a) {
-    extern __shared__ uint64_t lds[];
+    extern __shared__ __align__(16) uint64_t lds[];
+    __shared__ int32_t wsum[16];
     constexpr bool HAS_V = VTraits<VT>::has;
     constexpr bool REAL = VTraits<VT>::real;
     const int H = a.H2, T = H + 2, bd = blockDim.x, tid = threadIdx.x, part = blockIdx.x;
@@ -411,14 +528,13 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         misc[0] = 0;
         misc[1] = 0;
     }
-    // prefix of the region sizes of this partition over the phase-1 workgroups
-    if (tid == 0) {
-        int32_t run = 0;
-        for (int w = 0; w < a.W; w++) {
-            roff[w] = run;
-            run += (int32_t)a.r_count[(int64_t)w * a.P + part];
-        }
-        roff[a.W] = run;
+    // prefix of this partition's region sizes over the phase-1 workgroups (W <= blockDim)
+    {
+        const int32_t c = tid < a.W ? (int32_t)a.r_count[(int64_t)tid * a.P + part] : 0;
+        int32_t total;
+        const int32_t ex = block_excl_scan(c, wsum, &total);
+        if (tid < a.W) roff[tid] = ex;
+        if (tid == 0) roff[a.W] = total;
     }
     __syncthreads();
 
@@ -483,19 +599,38 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         __syncthreads();
     };
 
-    // heavy partials of this partition's heavy keys, summed over the phase-1 workgroups
+    // heavy partials of this partition's heavy keys, summed over the phase-1 workgroups: a
+    // thread owns one key and a strided subset of the workgroups, so neighbouring lanes read
+    // neighbouring keys of one partial row, and each thread adds one combined partial
     const int32_t hk0 = a.meta[2 + part], hk1 = a.meta[2 + part + 1];
-    const int64_t nh = (int64_t)(hk1 - hk0) * a.W;
+    const int32_t nk = hk1 - hk0;
     const int32_t stride_k = a.kmax + 2;
-    for (int64_t j = tid; j < nh; j += bd) {
-        const int32_t k = hk0 + (int32_t)(j / a.W), w = (int32_t)(j % a.W);
-        const int64_t at = (int64_t)w * stride_k + k;
-        const uint32_t rows = a.p_rows[at];
-        if (!rows) continue;
-        const int s = insert(a.heavy[k]);
-        if (s < 0) continue;
-        add(s, rows, a.p_cnt[at], tsum ? a.p_sum[at] : 0, tmin ? a.p_min[at] : ~0ull,
-            tmax ? a.p_max[at] : 0);
+    if (nk > 0) {
+        const int per = nk <= bd ? bd / nk : 1;  // threads per key
+        const int span = nk <= bd ? per * nk : bd;
+        if (tid < span) {
+            const int w0 = nk <= bd ? tid / nk : 0;
+            for (int ki = nk <= bd ? tid % nk : tid; ki < nk; ki += (nk <= bd ? nk : bd)) {
+                uint32_t rows = 0, cnt = 0;
+                uint64_t isum = 0, mn = ~0ull, mx = 0;
+                double dsum = 0.0;
+                for (int w = w0; w < a.W; w += per) {
+                    const int64_t at = (int64_t)w * stride_k + hk0 + ki;
+                    rows += a.p_rows[at];
+                    cnt += a.p_cnt[at];
+                    if (tsum) {
+                        if (REAL) dsum += __longlong_as_double((long long)a.p_sum[at]);
+                        else isum += a.p_sum[at];
+                    }
+                    if (tmin) mn = min(mn, (uint64_t)a.p_min[at]);
+                    if (tmax) mx = max(mx, (uint64_t)a.p_max[at]);
+                }
+                if (!rows) continue;
+                const int s = insert(a.heavy[hk0 + ki]);
+                if (s < 0) continue;
+                add(s, rows, cnt, REAL ? (uint64_t)__double_as_longlong(dsum) : isum, mn, mx);
+            }
+        }
     }
     if (part == 0) {  // NULL key and key == kEmpty: side slots
         for (int64_t j = tid; j < 2 * (int64_t)a.W; j += bd) {
@@ -508,19 +643,30 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         }
     }
 
-    // cold rows of this partition, in chunks that cannot overfill the LDS table
+    // cold rows of this partition, in passes that cannot overfill the LDS table: a pass takes
+    // at most as many rows as the table has room for new keys (merging first when that room
+    // is small), and a thread loads all of its rows of a pass before it inserts any
     const int64_t total = roff[a.W];
     const int64_t limit = (int64_t)H * 4 / 5;
-    int64_t chunk = std::min<int64_t>((int64_t)bd * 4, limit);
-    if (chunk < 1) chunk = 1;
-    for (int64_t base = 0; base < total; base += chunk) {
+    const int64_t step = std::min<int64_t>((int64_t)bd * kP2Rows, limit);
+    for (int64_t base = 0; base < total;) {
         __syncthreads();
         // every thread reads the fill level before any thread inserts again
-        const bool full = (int64_t)misc[0] + chunk > limit;
+        int64_t room = limit - (int64_t)misc[0];
         __syncthreads();
-        if (full) merge();
-        const int64_t end = std::min<int64_t>(base + chunk, total);
-        for (int64_t j = base + tid; j < end; j += bd) {
+        if (room < std::min<int64_t>(bd, limit)) {
+            merge();
+            room = limit;
+        }
+        const int64_t end = base + std::min<int64_t>(std::min<int64_t>(room, step), total - base);
+        uint64_t key[kP2Rows];
+        typename std::conditional<HAS_V, VT, int>::type val[kP2Rows];
+#pragma unroll
+        for (int i = 0; i < kP2Rows; i++) {
+            const int64_t j = base + (int64_t)i * bd + tid;
+            key[i] = kEmpty;
+            val[i] = {};
+            if (j >= end) continue;
             // region of row j: last w with roff[w] <= j
             int lo = 0, hi = a.W;  // roff[lo] <= j < roff[hi]
             while (hi - lo > 1) {
@@ -528,16 +674,22 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 if ((int64_t)roff[mid] <= j) lo = mid; else hi = mid;
             }
             const int64_t at = ((int64_t)lo * a.P + part) * (int64_t)a.cap + (j - roff[lo]);
-            const uint64_t key = a.r_keys[at];
-            const int s = insert(key);
+            key[i] = a.r_keys[at];
+            if constexpr (HAS_V) val[i] = ((const VT*)a.r_vals)[at];
+        }
+#pragma unroll
+        for (int i = 0; i < kP2Rows; i++) {
+            if (base + (int64_t)i * bd + tid >= end) continue;
+            const int s = insert(key[i]);
             if (s < 0) continue;
             if constexpr (HAS_V) {
-                const Part pr = row_part<VT>(((const VT*)a.r_vals)[at], false);
+                const Part pr = row_part<VT>(val[i], false);
                 add(s, 1, 1, pr.sum, pr.mn, pr.mx);
             } else {
                 add(s, 1, 1, 0, ~0ull, 0);
             }
         }
+        base = end;
     }
     merge();
     if (part == 0 && tid < 2) {
@@ -563,7 +715,7 @@ template <typename VT, int MODE>
 int launch_sample(TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc, const void* vals,
                   const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
                   int64_t n, int64_t nsamples, int64_t stride, uint64_t smask, hipStream_t s) {
-    k_tp_sample<VT, MODE><<<grid_cap(nsamples, 256, device_info().cus * 4), 256, 0, s>>>(
+    k_tp_sample<VT, MODE><<<grid_cap(nsamples, 1024, device_info().cus), 256, 0, s>>>(
         ks, kc, (const VT*)vals, vnull, mask, ps, vpred, n, stride, nsamples, b.sample_keys,
         b.sample_cnt, smask);
     QK_HIP(hipGetLastError());
@@ -581,11 +733,11 @@ int launch_sample_m(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols&
     }
 }
 
-template <typename VT, int MODE>
+template <typename VT, int MODE, typename KT>
 int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc, const void* vals,
               const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
               int64_t n, uint32_t fields, hipStream_t s) {
-    auto kern = k_tp_phase1<VT, MODE>;  // gfx950 launches up to 160 KB of LDS without an attribute
+    // gfx950 launches up to 160 KB of LDS per workgroup without a function attribute
     P1Args a;
     a.heavy = b.heavy_keys;
     a.meta = b.heavy_meta;
@@ -601,26 +753,41 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
     a.r_keys = b.r_keys;
     a.r_vals = b.r_vals;
     a.r_count = b.r_count;
-    const size_t lds = p1_lds_bytes(b.H1, b.P, fields);
-    kern<<<b.W, b.block, lds, s>>>(g, ks, kc, (const VT*)vals, vnull, mask, ps, vpred, n, fields, a);
+    a.zero = b.zero_word;
+    a.accumulate = b.pending_rows > 0 ? 1 : 0;
+    const size_t lds = p1_lds_bytes(b.H1, b.P, b.kmax, fields);
+    k_tp_phase1<VT, MODE, KT><<<b.W, b.block, lds, s>>>(g, ks, kc, (const VT*)vals, vnull, mask,
+                                                        ps, vpred, n, fields, a);
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }
 
-template <typename VT>
+template <typename VT, typename KT>
 int launch_p1_m(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
                 const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
                 uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
     switch (mode) {
-    case 0: return launch_p1<VT, 0>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    case 1: return launch_p1<VT, 1>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    default: return launch_p1<VT, 2>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 0: return launch_p1<VT, 0, KT>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 1: return launch_p1<VT, 1, KT>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    default: return launch_p1<VT, 2, KT>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     }
+}
+
+// Typed fast paths for one int64 or int32 key column; any other key layout reads its key
+// through the generic key spec.
+template <typename VT>
+int launch_p1_k(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
+                uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
+    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_LONG)
+        return launch_p1_m<VT, int64_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_INT)
+        return launch_p1_m<VT, int32_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    return launch_p1_m<VT, void>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
 }
 
 template <typename VT>
 int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
-    auto kern = k_tp_phase2<VT>;
     P2Args a;
     a.heavy = b.heavy_keys;
     a.meta = b.heavy_meta;
@@ -637,7 +804,7 @@ int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
     a.r_keys = b.r_keys;
     a.r_vals = b.r_vals;
     a.r_count = b.r_count;
-    kern<<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
+    k_tp_phase2<VT><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }
@@ -651,50 +818,65 @@ bool twophase_supported(int32_t kw, int32_t value_type) {
 
 void twophase_free(TwoPhaseBufs& b) {
     void* ptrs[] = {b.heavy_keys, b.heavy_meta, b.sample_keys, b.sample_cnt, b.hist, b.cand,
-                    b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_keys, b.r_vals, b.r_count};
+                    b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_keys, b.r_vals, b.r_count,
+                    b.zero_word};
     for (void* p : ptrs)
         if (p) pool_free(p);
     TwoPhaseBufs fresh;
+    fresh.defer_rows = b.defer_rows;  // a setting, not a buffer
     b = fresh;
 }
 
-int twophase_prepare(TwoPhaseBufs& b, const GView& g, int32_t parts, uint32_t fields,
-                     int32_t value_type, bool has_values, int64_t n, hipStream_t s) {
+namespace {
+
+// Rows the regions are sized for: one block, or several when phase 2 is deferred.
+int64_t region_rows_for(const TwoPhaseBufs& b, int64_t n, bool defer) {
+    if (!defer || b.defer_rows == 0) return n;
+    const int64_t want = b.defer_rows > 0 ? b.defer_rows : std::min<int64_t>(4 * n, (int64_t)1 << 28);
+    return std::max<int64_t>(n, std::min<int64_t>(want, (int64_t)1 << 30));
+}
+
+}  // namespace
+
+bool twophase_needs_realloc(const TwoPhaseBufs& b, int32_t parts, int32_t value_type,
+                            bool has_values, int64_t n, bool defer) {
+    const int vbytes = has_values ? type_size(value_type) : 0;
+    return !b.r_keys || b.block_alloc < n || b.vbytes != vbytes || b.P != parts ||
+           b.rows_alloc < region_rows_for(b, n, defer);
+}
+
+int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t value_type,
+                     bool has_values, int64_t n, bool defer, hipStream_t s) {
+    if (!twophase_needs_realloc(b, parts, value_type, has_values, n, defer)) return QK_SUCCESS;
     const DeviceInfo& d = device_info();
     const int vbytes = has_values ? type_size(value_type) : 0;
-    if (b.r_keys && b.rows_alloc >= n && b.vbytes == vbytes && b.P == parts) return QK_SUCCESS;
-    const bool had_heavy = b.heavy_ready;
+    const int64_t rows = region_rows_for(b, n, defer);
     twophase_free(b);
-    b.W = d.cus > 0 ? d.cus : 256;
+    b.W = d.cus > 0 ? std::min(d.cus, 1024) : 256;
     b.P = parts;
     b.vbytes = vbytes;
     b.block = kBlock;
-    const size_t lds = std::min<size_t>(d.lds_per_block, 160 * 1024);
-    // phase-1 LDS: at most half of the hash slots hold heavy keys
-    int32_t H1 = 64;
-    while (p1_lds_bytes(H1 * 2, parts, fields) <= lds) H1 *= 2;
-    int32_t lo = H1, hi = H1 * 2;
-    while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) / 2;
-        if (p1_lds_bytes(mid, parts, fields) <= lds) lo = mid; else hi = mid;
+    // static LDS of the kernels (a few words) stays outside this budget
+    const size_t lds = std::min<size_t>(d.lds_per_block, 160 * 1024) - 512;
+    // phase-1 LDS: buckets of kBucket slots, at most half of the slots hold heavy keys
+    int32_t H1 = 0;
+    for (int32_t h = kBucket; ; h += kBucket) {
+        if (p1_lds_bytes(h, parts, std::min(h / 2, 4096), fields) > lds) break;
+        H1 = h;
     }
-    b.H1 = lo;
-    b.kmax = std::min<int32_t>(b.H1 / 2, 4096);
+    b.H1 = H1;
+    b.kmax = std::min<int32_t>(H1 / 2, 4096);
     int32_t H2 = 64;
-    while (p2_lds_bytes(H2 * 2, b.W, fields) <= lds) H2 *= 2;
-    lo = H2;
-    hi = H2 * 2;
-    while (hi - lo > 1) {
-        const int32_t mid = (lo + hi) / 2;
-        if (p2_lds_bytes(mid, b.W, fields) <= lds) lo = mid; else hi = mid;
-    }
-    b.H2 = lo;
+    while (p2_lds_bytes(H2 + 64, b.W, fields) <= lds) H2 += 64;
+    b.H2 = H2;
     // the heavy keys of one partition must fit the phase-2 table below its 4/5 fill limit
     b.kmax = std::min<int32_t>(b.kmax, b.H2 * 3 / 5);
-    // regions: twice the mean rows per (workgroup, partition), at least 64
-    const int64_t mean = (n + (int64_t)b.W * parts - 1) / ((int64_t)b.W * parts);
-    b.cap = (int32_t)std::max<int64_t>(64, ((2 * mean + 32) + 31) / 32 * 32);
-    b.rows_alloc = n;
+    // regions: the mean rows per (workgroup, partition) and half again, plus a margin; rows
+    // beyond a full region take the atomic path, so this bounds speed, not correctness
+    const int64_t mean = (rows + (int64_t)b.W * parts - 1) / ((int64_t)b.W * parts);
+    b.cap = (int32_t)((mean + mean / 2 + 96 + 31) / 32 * 32);
+    b.rows_alloc = rows;
+    b.block_alloc = n;
     const int64_t sslots = (int64_t)1 << (b.sample_bits + 1);
     const int64_t kp = b.kmax + 2;
     const int64_t regions = (int64_t)b.W * parts * b.cap;
@@ -716,14 +898,15 @@ int twophase_prepare(TwoPhaseBufs& b, const GView& g, int32_t parts, uint32_t fi
     A((void**)&b.r_keys, (size_t)regions * 8);
     if (vbytes) A((void**)&b.r_vals, (size_t)regions * vbytes);
     A((void**)&b.r_count, (size_t)b.W * parts * 4);
+    A((void**)&b.zero_word, 64);
     if (st != QK_SUCCESS) {
         twophase_free(b);
         return st;
     }
     // no heavy set until the next sample: K = 0, every partition empty
     QK_HIP(hipMemsetAsync(b.heavy_meta, 0, (size_t)(parts + 8) * 4, s));
+    QK_HIP(hipMemsetAsync(b.zero_word, 0, 64, s));
     b.heavy_ready = false;
-    (void)had_heavy;
     return QK_SUCCESS;
 }
 
@@ -750,7 +933,7 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
     }
     if (st != QK_SUCCESS) return st;
     k_tp_hist<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots, b.hist);
-    k_tp_threshold<<<1, 64, 0, s>>>(b.hist, b.kmax, b.heavy_meta);
+    k_tp_threshold<<<1, kHistBins, 0, s>>>(b.hist, b.kmax, b.heavy_meta);
     k_tp_collect<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots,
                                                                b.kmax, b.heavy_meta, b.cand);
     k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.kmax, b.P, g.part_shift, b.heavy_meta, b.heavy_keys);
@@ -759,31 +942,47 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
     return QK_SUCCESS;
 }
 
-int twophase_block(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
-                   int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
-                   const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, int mode,
-                   hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+
+int twophase_phase1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                    int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
+                    const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, int mode,
+                    hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    if (b.pending_rows > 0 && (vals ? vtype : 0) != b.pend_vtype)
+        return fail(QK_ERROR, "group by: value column type changed between blocks");
+    if (b.pending_rows + n > b.rows_alloc || n > b.block_alloc)
+        return fail(QK_ERROR, "group by: two-phase buffers too small for the block");
     if (ev0) QK_HIP(hipEventRecord(ev0, s));
     int st;
     if (!vals) {
-        st = launch_p1_m<NoValue>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        st = launch_p1_k<NoValue>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     } else {
         switch (vtype) {
-        case QK_INT: st = launch_p1_m<int32_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
-        case QK_LONG: st = launch_p1_m<int64_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
-        case QK_FLOAT: st = launch_p1_m<float>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
-        default: st = launch_p1_m<double>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_INT: st = launch_p1_k<int32_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_LONG: st = launch_p1_k<int64_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_FLOAT: st = launch_p1_k<float>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        default: st = launch_p1_k<double>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
         }
     }
     if (st != QK_SUCCESS) return st;
     if (ev1) QK_HIP(hipEventRecord(ev1, s));
-    if (!vals) return launch_p2<NoValue>(b, g, fields, s);
-    switch (vtype) {
-    case QK_INT: return launch_p2<int32_t>(b, g, fields, s);
-    case QK_LONG: return launch_p2<int64_t>(b, g, fields, s);
-    case QK_FLOAT: return launch_p2<float>(b, g, fields, s);
-    default: return launch_p2<double>(b, g, fields, s);
+    b.pending_rows += n;
+    b.pend_vtype = vals ? vtype : 0;
+    return QK_SUCCESS;
+}
+
+int twophase_flush(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
+    if (b.pending_rows == 0) return QK_SUCCESS;
+    int st;
+    switch (b.pend_vtype) {
+    case 0: st = launch_p2<NoValue>(b, g, fields, s); break;
+    case QK_INT: st = launch_p2<int32_t>(b, g, fields, s); break;
+    case QK_LONG: st = launch_p2<int64_t>(b, g, fields, s); break;
+    case QK_FLOAT: st = launch_p2<float>(b, g, fields, s); break;
+    default: st = launch_p2<double>(b, g, fields, s); break;
     }
+    if (st != QK_SUCCESS) return st;
+    b.pending_rows = 0;
+    return QK_SUCCESS;
 }
 
 }  // namespace qk

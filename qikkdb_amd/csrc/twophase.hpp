@@ -8,6 +8,10 @@
 // into the partition's sub-table of the global table, which no other workgroup touches, so
 // the merge needs no atomics.  Rows that do not fit (NULL values, full regions) take the
 // atomic path in phase 1.
+//
+// Phase 2 may be deferred over several blocks: phase 1 then appends to the regions and adds
+// to the heavy partials, and one phase 2 merges all of them, so the global table is walked
+// once per query instead of once per block.  Whatever reads the table flushes first.
 #pragma once
 
 #include "groupby.hpp"
@@ -24,7 +28,11 @@ struct TwoPhaseBufs {
     int32_t H2 = 0;         // phase-2 LDS hash slots
     int32_t block = 1024;   // threads per workgroup (both phases)
     int32_t sample_bits = 18;
-    int64_t rows_alloc = 0; // block rows the regions were sized for
+    int64_t rows_alloc = 0;   // rows the regions were sized for (several blocks when deferring)
+    int64_t block_alloc = 0;  // largest block the buffers were sized for
+    int64_t defer_rows = -1;  // rows phase 2 may be deferred over: -1 default, 0 never
+    int64_t pending_rows = 0; // rows phase 1 processed since the last phase 2
+    int32_t pend_vtype = 0;   // value type of the pending rows (0 = no value column)
     // heavy set
     uint64_t* heavy_keys = nullptr;   // [kmax], sorted by partition
     int32_t* heavy_meta = nullptr;    // [0] = K, [1] = threshold, [2..] = start per partition (P+1)
@@ -42,6 +50,7 @@ struct TwoPhaseBufs {
     uint64_t* r_keys = nullptr;
     void* r_vals = nullptr;
     uint32_t* r_count = nullptr;      // [W][P]
+    uint64_t* zero_word = nullptr;    // 64 zero bytes
     bool heavy_ready = false;
 };
 
@@ -50,9 +59,13 @@ constexpr int kHistBins = 1024;
 // Whether the two-phase path applies: single-word keys and a value type it handles.
 bool twophase_supported(int32_t kw, int32_t value_type);
 
+// Whether the buffers must be reallocated for blocks of n rows (which drops pending rows).
+bool twophase_needs_realloc(const TwoPhaseBufs& b, int32_t parts, int32_t value_type,
+                            bool has_values, int64_t n, bool defer);
+
 // (Re)size the buffers for blocks of n rows; fields/value type decide the LDS layouts.
-int twophase_prepare(TwoPhaseBufs& b, const GView& g, int32_t parts, uint32_t fields,
-                     int32_t value_type, bool has_values, int64_t n, hipStream_t s);
+int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t value_type,
+                     bool has_values, int64_t n, bool defer, hipStream_t s);
 void twophase_free(TwoPhaseBufs& b);
 
 // Sample the block and build the heavy-hitter set (device-side, no host round trip).
@@ -60,10 +73,13 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
                     int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
                     const PredSet& ps, uint32_t vpred, int64_t n, int mode, hipStream_t s);
 
-// Phase 1 + phase 2 over one block.
-int twophase_block(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
-                   int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
-                   const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, int mode,
-                   hipStream_t s, hipEvent_t ev_p1_start, hipEvent_t ev_p1_end);
+// Phase 1 over one block: heavy rows into the partials, cold rows into the regions.
+int twophase_phase1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
+                    int32_t vtype, const void* vals, const uint64_t* vnull, const int8_t* mask,
+                    const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, int mode,
+                    hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_end);
+
+// Phase 2 over everything phase 1 left since the last flush (nothing to do if none).
+int twophase_flush(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s);
 
 }  // namespace qk

@@ -254,6 +254,17 @@ class GPUGroupBy:
         """-1 automatic, 0 one-pass kernel, 1 two-phase partitioned kernels."""
         call("qk_gb_set_path", self.h, path, min_rows)
 
+    def set_defer_rows(self, rows: int = -1) -> None:
+        """Rows whose partition merge an asynchronous table may defer (-1 default, 0 none)."""
+        call("qk_gb_set_defer_rows", self.h, rows)
+
+    def stats(self, stream=None) -> dict:
+        names = ["path", "heavy_keys", "heavy_threshold", "region_rows", "p1_workgroups",
+                 "partitions", "region_cap", "p1_lds_slots", "p2_lds_slots", "heavy_limit"]
+        v = (C.c_int64 * len(names))()
+        call("qk_gb_stats", self.h, v, len(names), stream)
+        return dict(zip(names, list(v)))
+
     def set_profiling(self, on: bool = True) -> None:
         call("qk_gb_set_profiling", self.h, 1 if on else 0)
 

@@ -36,7 +36,7 @@ class Where:
 
 def gpu_group_by(aggs, keys, value=None, key_nulls=None, value_nulls=None, where=(), mask=None,
                  block=None, capacity=1 << 16, flags=0, tuning=None, out_types=None,
-                 extra_cols=None, value_type=None, path=None):
+                 extra_cols=None, value_type=None, path=None, defer_rows=None):
     """keys: list of numpy arrays; key_nulls / value_nulls: bool arrays or None.
     Rows are processed in blocks of `block` rows (each block uploaded with its own masks)."""
     keys = [np.ascontiguousarray(k) for k in keys]
@@ -49,6 +49,8 @@ def gpu_group_by(aggs, keys, value=None, key_nulls=None, value_nulls=None, where
         gb.set_tuning(**tuning)
     if path is not None:
         gb.set_path(path, 0)
+    if defer_rows is not None:
+        gb.set_defer_rows(defer_rows)
     block = block or n
     extra_cols = extra_cols or {}
     for s in range(0, n, block):

@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 import qikkdb_amd as qk
 from qikkdb_amd import datagen
-from gpu_util import Where, cpu_group_by, gpu_group_by
+from gpu_util import Where, col, cpu_group_by, gpu_group_by
 from helpers import AGG_NAMES, as_map, expect_map, golden, maps_close
 
 pytestmark = pytest.mark.gpu
@@ -327,3 +327,43 @@ def test_two_phase_table_full_reported():
     with pytest.raises(qk.QueryEngineError) as e:
         gpu_group_by(qk.AGG_COUNT, [keys], None, capacity=10_000, path=1, value_type=qk.INT)
     assert e.value.code == qk.QK_HASH_TABLE_FULL
+
+
+@pytest.mark.parametrize("defer_rows", [-1, 0, 3 << 18, 1 << 18])
+def test_two_phase_deferred_merge(defer_rows):
+    # an asynchronous table keeps the partitioned rows of several blocks and merges them
+    # when read; a budget that fills mid-query forces merges between blocks
+    rng = np.random.default_rng(41)
+    n = 3_000_000
+    keys = datagen.zipf_keys(12, 1.1, 150_000, n)
+    vals = rng.lognormal(0, 1, n).astype(np.float32)
+    vn = rng.random(n) < 0.05
+    kn = [rng.random(n) < 0.002]
+    w = [Where(qk.GREATER_EQUAL, "value", 0.5, qk.FLOAT), Where(qk.LESS_EQUAL, "value", 4.0, qk.FLOAT)]
+    aggs = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT_ALL]
+    gpu = gpu_group_by(aggs, [keys], vals, kn, vn, where=w, block=1 << 18, capacity=200_000,
+                       path=1, flags=qk.GB_ASYNC, defer_rows=defer_rows)
+    cpu = cpu_group_by(aggs, [keys], vals, kn, vn, where=w)
+    compare(gpu, cpu, len(aggs), real_aggs=(0, 1, 3, 4))
+
+
+def test_two_phase_reset_drops_pending_rows():
+    rng = np.random.default_rng(43)
+    n = 1 << 20
+    ka = rng.integers(0, 5000, n).astype(np.int64)
+    kb = rng.integers(10_000, 12_000, n).astype(np.int64)
+    va = rng.integers(-1000, 1000, n).astype(np.int64)
+    vb = rng.integers(-1000, 1000, n).astype(np.int64)
+    aggs = [qk.AGG_SUM, qk.AGG_COUNT]
+    gb = qk.GPUGroupBy(aggs, qk.LONG, qk.LONG, 20_000, 1, flags=qk.GB_ASYNC)
+    gb.set_path(1, 0)
+    bufs = []
+    for k, v in ((ka, va), (kb, vb)):
+        kc, vc = col(k), col(v)
+        bufs.append((kc, vc))
+        gb.ProcessBlock([kc.data], vc.data, n)
+        if k is ka:
+            gb.reset()
+    got = gb.results_host()
+    gb.close()
+    compare(got, cpu_group_by(aggs, [kb], vb), 2)
