@@ -1,13 +1,18 @@
 """Benchmark: filtered GROUP BY rows/s on one MI355X, with the roofline of the dominant kernel
 and the CPU oracle timed on the host as a baseline.
 
-Workload (the single-GPU configuration the metric is quoted on): 256M rows in four 2^26-row
-blocks; key int64 Zipf(s=1.1) over 1e6 distinct values; value float32 lognormal(0,1) with 5%
-NULL values; WHERE value BETWEEN 0.5 AND 4.0; GROUP BY key with SUM / AVG / COUNT.  Columns
-are generated on the host and copied to HBM before the timed region.  One step = clear the
-table, four fused WHERE+GROUP BY block launches, GetResults into device buffers.
+Default workload (the single-GPU configuration the metric is quoted on): 256M rows in four
+2^26-row blocks; key int64 Zipf(s=1.1) over 1e6 distinct values; value float32
+lognormal(0,1) with 5% NULL values; WHERE value BETWEEN 0.5 AND 4.0; GROUP BY key with
+SUM / AVG / COUNT.  `--workload composite` runs the composite-key case instead: 512M rows of
+(a int32, b int32) keys over 16.7M possible groups, WHERE ts in a 30-day window of one year,
+SUM / MIN / MAX / COUNT of an int64 value.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+Columns are generated on the host and copied to HBM before the timed region.  One step =
+clear the table, one fused WHERE+GROUP BY launch sequence per block, GetResults into device
+buffers.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload zipf|composite]
 """
 from __future__ import annotations
 
@@ -24,7 +29,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "filtered group-by rows/sec and % HBM peak at 256M rows; batched queries/sec at p99"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_ROW = 8 + 4 + 1.0 / 8  # int64 key + float32 value + value null bit
 
 
 def log(msg: str) -> None:
@@ -32,7 +36,7 @@ def log(msg: str) -> None:
 
 
 def read_traffic(path: str):
-    """HBM bytes per launch of the row kernel from the committed rocprof counter summary
+    """HBM bytes per launch of the dominant kernel from the committed rocprof counter summary
     (a line 'traffic_bytes_per_launch: N'), or None."""
     try:
         with open(path) as f:
@@ -44,21 +48,116 @@ def read_traffic(path: str):
     return None
 
 
-def cpu_baseline(d: dict, rows: int, repeats: int = 2) -> dict:
+class Zipf:
+    """int64 Zipf key, f32 lognormal value with NULLs, WHERE value BETWEEN 0.5 AND 4.0."""
+    name = "zipf"
+    rows = 1 << 28
+    bytes_per_row = 8 + 4 + 1.0 / 8  # int64 key + float32 value + value null bit
+    description = ("zipf-256M: int64 key Zipf(s=1.1) over 1e6 distinct, f32 lognormal(0,1) "
+                   "value with 5% NULLs, WHERE value BETWEEN 0.5 AND 4.0, SUM/AVG/COUNT")
+    dtype = "f32"
+    counters = "k_tp_phase1_zipf256M_counters.txt"
+
+    def __init__(self, qk, n):
+        from qikkdb_amd import datagen
+        self.qk, self.n = qk, n
+        self.d = datagen.zipf(n, seed=2)
+        self.key = qk.DeviceBuffer.from_numpy(self.d["key"])
+        self.val = qk.DeviceBuffer.from_numpy(self.d["value"])
+        self.vnull = qk.DeviceBuffer.from_numpy(self.d["value_nulls"])
+        self.aggs = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT]
+
+    def table(self):
+        qk = self.qk
+        return qk.GPUGroupBy(self.aggs, qk.LONG, qk.FLOAT, 1 << 20, 1,
+                             flags=qk.GB_KEYS_NOT_NULL | qk.GB_ASYNC)
+
+    def block(self, off, blk):
+        qk = self.qk
+        vp = self.val.ptr + off * 4
+        np_ = self.vnull.ptr + (off // 64) * 8
+        vcol = qk.DeviceColumn(qk.DeviceBuffer(blk * 4, ptr=vp, owned=False), qk.FLOAT, blk,
+                               qk.DeviceBuffer(blk // 8, ptr=np_, owned=False))
+        preds = [qk.where(qk.GREATER_EQUAL, vcol, qk.Const(0.5, qk.FLOAT)),
+                 qk.where(qk.LESS_EQUAL, vcol, qk.Const(4.0, qk.FLOAT))]
+        return [self.key.ptr + off * 8], vp, np_, preds
+
+    def passing(self):
+        v = self.d["value"]
+        isnull = np.unpackbits(self.d["value_nulls"].view(np.uint8), bitorder="little")[:self.n].astype(bool)
+        return int(np.count_nonzero((v >= np.float32(0.5)) & (v <= np.float32(4.0)) & ~isnull))
+
+    def count_column(self):
+        return 2
+
+    def cpu_query(self, O, rows, nthreads):
+        keys = self.d["key"][:rows]
+        vals = self.d["value"][:rows]
+        vnull = self.d["value_nulls"][: (rows + 63) // 64]
+        preds = [O.pred(1, vals, 0.5, O.FLOAT, vnull), O.pred(3, vals, 4.0, O.FLOAT, vnull)]
+        return lambda: O.query_mt(nthreads, [O.AGG_SUM, O.AGG_AVG, O.AGG_COUNT], O.LONG, O.FLOAT,
+                                  [keys], vals, vnull, preds=preds)
+
+
+class Composite:
+    """(a int32, b int32) key, WHERE ts in a 30-day window, SUM/MIN/MAX/COUNT of int64."""
+    name = "composite"
+    rows = 1 << 29
+    bytes_per_row = 4 + 4 + 8 + 8  # a + b + ts + value
+    description = ("composite-512M: keys (int32 a, int32 b) uniform [0,4096)^2, WHERE int64 ts in "
+                   "a 30-day window of one year, SUM/MIN/MAX/COUNT of int64 value")
+    dtype = "int64"
+    counters = "k_tp_phase1_composite512M_counters.txt"
+
+    def __init__(self, qk, n):
+        from qikkdb_amd import datagen
+        self.qk, self.n, self.dg = qk, n, datagen
+        self.d = datagen.composite(n, seed=3)
+        self.cols = {k: qk.DeviceBuffer.from_numpy(v) for k, v in self.d.items()}
+        self.aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+
+    def table(self):
+        qk = self.qk
+        return qk.GPUGroupBy(self.aggs, [qk.INT, qk.INT], qk.LONG, 1 << 24, 1,
+                             flags=qk.GB_KEYS_NOT_NULL | qk.GB_ASYNC)
+
+    def block(self, off, blk):
+        qk = self.qk
+        ts = qk.DeviceColumn(qk.DeviceBuffer(blk * 8, ptr=self.cols["ts"].ptr + off * 8, owned=False),
+                             qk.LONG, blk)
+        preds = [qk.where(qk.GREATER_EQUAL, ts, qk.Const(self.dg.WINDOW_START_MS, qk.LONG)),
+                 qk.where(qk.LESS, ts, qk.Const(self.dg.WINDOW_END_MS, qk.LONG))]
+        return ([self.cols["a"].ptr + off * 4, self.cols["b"].ptr + off * 4],
+                self.cols["value"].ptr + off * 8, None, preds)
+
+    def passing(self):
+        ts = self.d["ts"]
+        return int(np.count_nonzero((ts >= self.dg.WINDOW_START_MS) & (ts < self.dg.WINDOW_END_MS)))
+
+    def count_column(self):
+        return 3
+
+    def cpu_query(self, O, rows, nthreads):
+        a, b = self.d["a"][:rows], self.d["b"][:rows]
+        ts, v = self.d["ts"][:rows], self.d["value"][:rows]
+        preds = [O.pred(1, ts, self.dg.WINDOW_START_MS, O.LONG), O.pred(2, ts, self.dg.WINDOW_END_MS, O.LONG)]
+        return lambda: O.query_mt(nthreads, [O.AGG_SUM, O.AGG_MIN, O.AGG_MAX, O.AGG_COUNT],
+                                  [O.INT, O.INT], O.LONG, [a, b], v, preds=preds)
+
+
+WORKLOADS = {"zipf": Zipf, "composite": Composite}
+
+
+def cpu_baseline(w, rows: int, repeats: int = 2) -> dict:
     import oracle as O
 
     nthreads = max(1, min(16, os.cpu_count() or 1))
-    keys = d["key"][:rows]
-    vals = d["value"][:rows]
-    vnull = d["value_nulls"][: (rows + 63) // 64]
-    preds = [O.pred(1, vals, 0.5, O.FLOAT, vnull), O.pred(3, vals, 4.0, O.FLOAT, vnull)]
-    aggs = [O.AGG_SUM, O.AGG_AVG, O.AGG_COUNT]
+    run = w.cpu_query(O, rows, nthreads)
     best = float("inf")
     for _ in range(repeats):
         t0 = time.perf_counter()
-        g = O.query_mt(nthreads, aggs, O.LONG, O.FLOAT, [keys], vals, vnull, preds=preds)
-        dt = time.perf_counter() - t0
-        best = min(best, dt)
+        g = run()
+        best = min(best, time.perf_counter() - t0)
         del g
     return {"value": rows / best, "unit": "rows/s", "cores": nthreads, "kind": "port",
             "sample": f"first {rows} rows of the same workload (one table per thread, merged), "
@@ -70,13 +169,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rows", type=int, default=1 << 28)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="zipf")
+    ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's size)")
     ap.add_argument("--block-rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-rows", type=int, default=1 << 25)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--block", type=int, default=-1, help="threads per workgroup of the row kernel")
-    ap.add_argument("--lds-bytes", type=int, default=-1, help="LDS table bytes per workgroup")
-    ap.add_argument("--wgs-per-cu", type=int, default=-1)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one plain device-to-device copy of this size before the steps "
                          "(a known-size reference for HBM byte counters)")
@@ -89,21 +186,17 @@ def main() -> int:
         return 2
 
     import qikkdb_amd as qk
-    from qikkdb_amd import datagen
 
     qk.ensure_device(0)
-    n, blk = args.rows, args.block_rows
+    W = WORKLOADS[args.workload]
+    n, blk = args.rows or W.rows, args.block_rows
     assert n % blk == 0 and blk % 64 == 0
     nblocks = n // blk
 
     t0 = time.perf_counter()
-    d = datagen.zipf(n, seed=2)
-    log(f"generated {n} rows in {time.perf_counter() - t0:.1f} s")
-    t0 = time.perf_counter()
-    key = qk.DeviceBuffer.from_numpy(d["key"])
-    val = qk.DeviceBuffer.from_numpy(d["value"])
-    vnull = qk.DeviceBuffer.from_numpy(d["value_nulls"])
-    log(f"copied {n * BYTES_PER_ROW / 1e9:.2f} GB to HBM in {time.perf_counter() - t0:.1f} s")
+    w = W(qk, n)
+    log(f"generated {n} rows and copied {n * W.bytes_per_row / 1e9:.2f} GB to HBM in "
+        f"{time.perf_counter() - t0:.1f} s")
     if args.copy_bytes > 0:
         src = qk.DeviceBuffer(args.copy_bytes)
         dst = qk.DeviceBuffer(args.copy_bytes)
@@ -112,23 +205,8 @@ def main() -> int:
         log(f"reference copy of {args.copy_bytes} bytes done")
         del src, dst
 
-    aggs = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT]
-    gb = qk.GPUGroupBy(aggs, qk.LONG, qk.FLOAT, 1 << 20, 1,
-                       flags=qk.GB_KEYS_NOT_NULL | qk.GB_ASYNC)
-    gb.set_tuning(args.block, args.lds_bytes, args.wgs_per_cu)
-
-    def block_args(b):
-        off = b * blk
-        kp = key.ptr + off * 8
-        vp = val.ptr + off * 4
-        np_ = vnull.ptr + (off // 64) * 8
-        vcol = qk.DeviceColumn(qk.DeviceBuffer(blk * 4, ptr=vp, owned=False), qk.FLOAT, blk,
-                               qk.DeviceBuffer(blk // 8, ptr=np_, owned=False))
-        preds = [qk.where(qk.GREATER_EQUAL, vcol, qk.Const(0.5, qk.FLOAT)),
-                 qk.where(qk.LESS_EQUAL, vcol, qk.Const(4.0, qk.FLOAT))]
-        return [kp], vp, np_, preds
-
-    blocks = [block_args(b) for b in range(nblocks)]
+    gb = w.table()
+    blocks = [w.block(b * blk, blk) for b in range(nblocks)]
 
     def step():
         gb.reset()
@@ -151,24 +229,24 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     kern_ms, launches = gb.profile()
     gb.set_profiling(False)
+    stats = gb.stats()
+    kernel = "k_tp_phase1" if stats["path"] == 1 else "k_gb_rows"
     ms_per_step = elapsed * 1e3 / args.steps
     rows_per_s = n * args.steps / elapsed
     avg_launch_ms = kern_ms / max(1, launches)
-    achieved = blk * BYTES_PER_ROW / (avg_launch_ms * 1e-3) / 1e9
-    log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, row kernel {avg_launch_ms:.3f} ms per "
+    achieved = blk * W.bytes_per_row / (avg_launch_ms * 1e-3) / 1e9
+    log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {kernel} {avg_launch_ms:.3f} ms per "
         f"{blk}-row launch over {launches} launches, {achieved:.0f} GB/s algorithmic")
-    log(f"table state: {gb.stats()}")
+    log(f"table state: {stats}")
 
     # cheap consistency check of the last step: COUNT sums to the rows that pass the WHERE
-    cnt = r.values[2].to_numpy(np.int64, r.count)
-    v = d["value"]
-    isnull = np.unpackbits(d["value_nulls"].view(np.uint8), bitorder="little")[:n].astype(bool)
-    passed = int(np.count_nonzero((v >= np.float32(0.5)) & (v <= np.float32(4.0)) & ~isnull))
+    cnt = r.values[w.count_column()].to_numpy(np.int64, r.count)
+    passed = w.passing()
     if int(cnt.sum()) != passed:
         log(f"CHECK FAILED: counts sum to {int(cnt.sum())}, {passed} rows pass the filter")
         return 1
 
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "k_tp_phase1_zipf256M_counters.txt"))
+    traffic = read_traffic(os.path.join(ROOT, "profiles", W.counters))
     out = {
         "metric": METRIC,
         "value": rows_per_s,
@@ -180,13 +258,12 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": W.dtype,
         "data": "synthetic",
         "config": {
-            "workload": "zipf-256M: int64 key Zipf(s=1.1) over 1e6 distinct, f32 lognormal(0,1) "
-                        "value with 5% NULLs, WHERE value BETWEEN 0.5 AND 4.0, SUM/AVG/COUNT",
+            "workload": W.description,
             "rows": n, "block_rows": blk, "blocks": nblocks, "groups": groups,
-            "accumulator": "f64", "passed_rows": passed,
+            "accumulator": "f64" if W.dtype == "f32" else "int64", "passed_rows": passed,
         },
         "roofline": {
             "bound": "hbm",
@@ -195,16 +272,16 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_tp_phase1<float,2>",
+            "kernel": kernel,
             "avg_launch_ms": avg_launch_ms,
-            "algorithmic_bytes_per_launch": blk * BYTES_PER_ROW,
+            "algorithmic_bytes_per_launch": blk * W.bytes_per_row,
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         rows = min(args.cpu_rows, n)
         log(f"timing the CPU oracle on {rows} rows")
-        out["cpu_baseline"] = cpu_baseline(d, rows)
+        out["cpu_baseline"] = cpu_baseline(w, rows)
     gb.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -90,9 +90,27 @@ def build_datagen(verbose: bool = False) -> str:
     return DATAGEN_LIB
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "groupby_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "groupby_driver")
+
+
+def build_driver(verbose: bool = False) -> str | None:
+    """The C++ caller of the drop-in headers (include/qikkdb/) used by the parity tests."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+          "-L", PKG, "-lqikkdb_amd", "-Wl,-rpath,$ORIGIN/../../qikkdb_amd", "-o", tmp])
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 def build(force: bool = True, jobs: int = 4, verbose: bool = False) -> None:
     build_datagen(verbose)
     build_hip(force=force, jobs=jobs, verbose=verbose)
+    build_driver(verbose)
 
 
 if __name__ == "__main__":

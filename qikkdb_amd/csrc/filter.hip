@@ -6,6 +6,9 @@
 // the bit form packs 64 rows per uint64 word with one wavefront ballot per 64 rows.
 #include <cstring>
 
+#include <cmath>
+#include <cstdint>
+
 #include "pred.hpp"
 
 namespace qk {
@@ -202,6 +205,103 @@ int check_preds(int32_t npreds, const qk_predicate* preds, PredSet* out) {
         out->p[i] = p;
     }
     return QK_SUCCESS;
+}
+
+namespace {
+
+// One bound of one domain while folding: value and whether it is exclusive.
+template <typename T>
+struct Bound {
+    T v;
+    bool strict;
+};
+
+template <typename T>
+void tighten(int op, T k, Bound<T>& lo, Bound<T>& hi) {
+    const bool gt = op == QK_GREATER, ge = op == QK_GREATER_EQUAL || op == QK_EQUAL;
+    const bool lt = op == QK_LESS, le = op == QK_LESS_EQUAL || op == QK_EQUAL;
+    if ((gt || ge) && (k > lo.v || (k == lo.v && gt))) lo = {k, gt};
+    if ((lt || le) && (k < hi.v || (k == hi.v && lt))) hi = {k, lt};
+}
+
+struct Fold {
+    int32_t doms = 0;
+    Bound<int64_t> lo_i{INT64_MIN, false}, hi_i{INT64_MAX, false};
+    Bound<float> lo_f{-INFINITY, false}, hi_f{INFINITY, false};
+    Bound<double> lo_d{-INFINITY, false}, hi_d{INFINITY, false};
+};
+
+// One comparison `v op c` with v of type VT and c of type R, in their common type.
+template <typename VT, typename R>
+void fold_one(int op, R c, Fold& f) {
+    using D = typename std::common_type<VT, R>::type;
+    const D k = (D)c;
+    if constexpr (std::is_integral<D>::value) {
+        f.doms |= 1;
+        tighten<int64_t>(op, (int64_t)k, f.lo_i, f.hi_i);
+    } else if constexpr (std::is_same<D, float>::value) {
+        f.doms |= 2;
+        tighten<float>(op, k, f.lo_f, f.hi_f);
+    } else {
+        f.doms |= 4;
+        tighten<double>(op, (double)k, f.lo_d, f.hi_d);
+    }
+}
+
+template <typename VT>
+void fold_typed(const qk_predicate& p, Fold& f) {
+    switch (p.rhs_type) {
+    case QK_INT8: fold_one<VT, int8_t>(p.op, (int8_t)p.rhs_int, f); break;
+    case QK_INT: fold_one<VT, int32_t>(p.op, (int32_t)p.rhs_int, f); break;
+    case QK_LONG: fold_one<VT, int64_t>(p.op, (int64_t)p.rhs_int, f); break;
+    case QK_FLOAT: fold_one<VT, float>(p.op, (float)p.rhs_real, f); break;
+    default: fold_one<VT, double>(p.op, p.rhs_real, f); break;
+    }
+}
+
+// A floating bound as the pair the kernels test (the unused member NaN).
+template <typename T>
+void split(const Bound<T>& b, T& x, T& i) {
+    x = b.strict ? b.v : (T)NAN;
+    i = b.strict ? (T)NAN : b.v;
+}
+
+}  // namespace
+
+bool fold_value_range(PredSet* ps, uint32_t vpred, int32_t vtype) {
+    if (ps->n == 0) return false;
+    Fold f;
+    for (int i = 0; i < ps->n; i++) {
+        const qk_predicate& p = ps->p[i];
+        if (!((vpred >> i) & 1u) || p.rhs_kind != QK_CONST) return false;
+        if (p.op != QK_GREATER && p.op != QK_GREATER_EQUAL && p.op != QK_LESS &&
+            p.op != QK_LESS_EQUAL && p.op != QK_EQUAL)
+            return false;
+        switch (vtype) {
+        case QK_INT: fold_typed<int32_t>(p, f); break;
+        case QK_LONG: fold_typed<int64_t>(p, f); break;
+        case QK_FLOAT: fold_typed<float>(p, f); break;
+        case QK_DOUBLE: fold_typed<double>(p, f); break;
+        default: return false;
+        }
+    }
+    ValueRange vr{};
+    vr.doms = f.doms;
+    // integer bounds inclusive; a strict bound at the end of the range empties the interval
+    vr.lo_i = f.lo_i.v;
+    vr.hi_i = f.hi_i.v;
+    if (f.lo_i.strict) {
+        if (vr.lo_i == INT64_MAX) { vr.lo_i = INT64_MAX; vr.hi_i = INT64_MIN; } else vr.lo_i++;
+    }
+    if (f.hi_i.strict) {
+        if (vr.hi_i == INT64_MIN) { vr.lo_i = INT64_MAX; vr.hi_i = INT64_MIN; } else vr.hi_i--;
+    }
+    split(f.lo_f, vr.lo_fx, vr.lo_fi);
+    split(f.hi_f, vr.hi_fx, vr.hi_fi);
+    split(f.lo_d, vr.lo_dx, vr.lo_di);
+    split(f.hi_d, vr.hi_dx, vr.hi_di);
+    ps->vr = vr;
+    return true;
 }
 
 }  // namespace qk

@@ -160,6 +160,10 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
                 if (MODE == 1) act[k] = mask[r] != 0;
             }
         }
+        if constexpr (MODE == 3 && HAS_V) {
+#pragma unroll
+            for (int k = 0; k < kSteps; k++) act[k] = act[k] && !vn[k] && in_range<VT>(preds.vr, val[k]);
+        }
         if (MODE == 2) {
 #pragma unroll
             for (int k = 0; k < kSteps; k++) {
@@ -458,7 +462,11 @@ int launch_rows_m(int mode, const GView& g, const KeySpec& ks, const KeyCols& kc
     switch (mode) {
     case 0: return launch_rows_t<KW, VT, 0>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
     case 1: return launch_rows_t<KW, VT, 1>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
-    default: return launch_rows_t<KW, VT, 2>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    case 2: return launch_rows_t<KW, VT, 2>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    default:
+        if constexpr (VTraits<VT>::has)
+            return launch_rows_t<KW, VT, 3>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+        return fail(QK_ERROR, "value range filter without a value column");
     }
 }
 
@@ -520,7 +528,7 @@ namespace qk {
 namespace {
 
 GView view_of(const qk_groupby_s* h) {
-    GView g;
+    GView g{};  // zeroed padding: views are compared bytewise
     g.keys = h->keys;
     g.cnt = h->cnt;
     g.cntall = h->cntall;
@@ -639,6 +647,8 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
         if (values && ps.p[p].column == values && ps.p[p].type == h->value_type &&
             ps.p[p].null_mask == value_nulls)
             vpred |= 1u << p;
+    // WHERE on the value column alone: test one interval per row instead of the list
+    if (mode == 2 && values && fold_value_range(&ps, vpred, h->value_type)) mode = 3;
     hipStream_t s = as_stream(stream);
     GView g = view_of(h);
     std::pair<hipEvent_t, hipEvent_t>* ev = h->profiling ? next_events(h) : nullptr;

@@ -77,14 +77,56 @@ __device__ __forceinline__ bool pred_on_value(const qk_predicate& p, V a, int64_
     }
 }
 
+// A conjunction of comparisons of the value column with constants, folded into one interval
+// per comparison domain: the type both operands convert to under the usual arithmetic
+// conversions (i64 for two integer types, else f32 or f64).  Comparing in that domain is
+// what each predicate does, so the fold is exact.  NOT_EQUAL and the logical operators do
+// not fold.
+//
+// Each floating bound is a pair: x passes the lower bound when x > lo_x or x >= lo_i, where
+// the unused member is NaN (no x compares true with NaN), so strictness needs no flag.  An
+// unused domain holds [-inf, +inf]; that only fails NaN, which fails some folded predicate
+// anyway.  Integer bounds are inclusive (a strict bound moves by one; an empty interval has
+// lo > hi).
+struct ValueRange {
+    int32_t doms;  // domains holding a bound: bit 0 i64, bit 1 f32, bit 2 f64
+    int64_t lo_i, hi_i;
+    float lo_fx, lo_fi, hi_fx, hi_fi;
+    double lo_dx, lo_di, hi_dx, hi_di;
+};
+
+template <typename T>
+__device__ __forceinline__ bool in_bounds(T x, T lo_x, T lo_i, T hi_x, T hi_i) {
+    return ((x > lo_x) | (x >= lo_i)) & ((x < hi_x) | (x <= hi_i));
+}
+
+template <typename VT>
+__device__ __forceinline__ bool in_range(const ValueRange& r, VT v) {
+    bool ok = true;
+    if constexpr (std::is_integral<VT>::value) ok = ((int64_t)v >= r.lo_i) & ((int64_t)v <= r.hi_i);
+    if constexpr (std::is_same<VT, float>::value) {
+        ok = in_bounds<float>(v, r.lo_fx, r.lo_fi, r.hi_fx, r.hi_fi);
+    } else if constexpr (!std::is_same<VT, double>::value) {
+        if (r.doms & 2) ok = ok & in_bounds<float>((float)v, r.lo_fx, r.lo_fi, r.hi_fx, r.hi_fi);
+    }
+    if (r.doms & 4) ok = ok & in_bounds<double>((double)v, r.lo_dx, r.lo_di, r.hi_dx, r.hi_di);
+    return ok;
+}
+
 // Predicates travel to kernels by value.
 constexpr int kMaxPreds = 8;
 struct PredSet {
     int32_t n;
     qk_predicate p[kMaxPreds];
+    ValueRange vr;  // the predicates folded, when they all fold (see fold_value_range)
 };
 
 // Host-side validation of a predicate array; returns QK_SUCCESS or an error.
 int check_preds(int32_t npreds, const qk_predicate* preds, PredSet* out);
+
+// Fold the predicates into ps->vr when every one compares the value column (vpred has a bit
+// per such predicate) of type vtype with a constant by >, >=, <, <= or ==.  Returns whether
+// it did; the fused kernels then test the interval instead of the predicate list.
+bool fold_value_range(PredSet* ps, uint32_t vpred, int32_t vtype);
 
 }  // namespace qk

@@ -5,6 +5,7 @@
 // histogram so that at most kmax keys qualify) become the heavy set, sorted by partition.
 // All of it runs on the device.
 #include <algorithm>
+#include <cstring>
 
 #include "twophase.hpp"
 
@@ -13,7 +14,7 @@ namespace qk {
 namespace {
 
 constexpr int kP1Steps = 4;   // rows per thread per tile in phase 1
-constexpr int kP2Rows = 4;    // cold rows per thread per chunk in phase 2
+constexpr int kP2Rows = 8;    // cold rows per thread per pass in phase 2
 constexpr int kBlock = 1024;
 constexpr int kBucket = 4;    // slots per bucket of the phase-1 heavy table
 constexpr int kSampleLds = 2048;
@@ -23,6 +24,7 @@ template <typename VT, int MODE, typename VV>
 __device__ __forceinline__ bool row_passes(const PredSet& preds, uint32_t vpred, const int8_t* mask,
                                            int64_t r, const VV& v, bool vn) {
     if (MODE == 1) return mask[r] != 0;
+    if constexpr (MODE == 3 && VTraits<VT>::has) return !vn && in_range<VT>(preds.vr, v);
     if (MODE == 2) {
         for (int p = 0; p < preds.n; p++) {
             const qk_predicate& q = preds.p[p];
@@ -174,21 +176,26 @@ __global__ void __launch_bounds__(kHistBins) k_tp_threshold(const uint32_t* hist
 
 __global__ void __launch_bounds__(256) k_tp_collect(const uint64_t* __restrict__ skeys,
                                                     const uint32_t* __restrict__ scnt, uint64_t slots,
-                                                    int32_t kmax, int32_t* meta, uint64_t* cand) {
+                                                    int32_t kmax, int32_t* meta, uint64_t* cand,
+                                                    uint32_t* cand_cnt) {
     const uint32_t t = (uint32_t)meta[1];
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride) {
         if (skeys[s] != kEmpty && scnt[s] >= t) {
             int32_t i = atomicAdd(meta, 1);
-            if (i < kmax) cand[i] = skeys[s];
+            if (i < kmax) {
+                cand[i] = skeys[s];
+                cand_cnt[i] = scnt[s];
+            }
         }
     }
 }
 
 // One workgroup: order the heavy candidates by partition; meta[2 + p] = first index of p.
-__global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ cand, int32_t kmax,
+__global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ cand,
+                                                  const uint32_t* __restrict__ cand_cnt, int32_t kmax,
                                                   int32_t P, int32_t part_shift, int32_t* meta,
-                                                  uint64_t* heavy) {
+                                                  uint64_t* heavy, uint32_t* heavy_cnt) {
     __shared__ int32_t cnt[1024];
     __shared__ int32_t cur[1024];
     __shared__ int32_t wsum[16];
@@ -216,7 +223,9 @@ __global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ c
     for (int i = t; i < K; i += blockDim.x) {
         const uint64_t h = hash64(cand[i]);
         const int p = part_shift >= 64 ? 0 : (int)(h >> part_shift);
-        heavy[atomicAdd(&cur[p], 1)] = cand[i];
+        const int32_t at = atomicAdd(&cur[p], 1);
+        heavy[at] = cand[i];
+        heavy_cnt[at] = cand_cnt[i];
     }
 }
 
@@ -224,6 +233,7 @@ __global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ c
 
 struct P1Args {
     const uint64_t* heavy;
+    const uint32_t* heavy_cnt;  // sample count of each heavy key
     const int32_t* meta;
     int32_t kmax, H, P, cap;
     uint32_t* p_cnt;
@@ -235,6 +245,7 @@ struct P1Args {
     void* r_vals;
     uint32_t* r_count;
     const uint64_t* zero;  // a zero word (stands in for an absent null bitmap)
+    int32_t part_shift;    // partition of a key hash: h >> part_shift (0 when >= 64)
     int32_t accumulate;    // regions and partials hold rows of earlier blocks
 };
 
@@ -253,6 +264,7 @@ __host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, int32_t kma
 __device__ __forceinline__ void buckets_of(uint64_t h, uint32_t nb, uint32_t& b1, uint32_t& b2) {
     b1 = (uint32_t)(((h >> 32) * (uint64_t)nb) >> 32);
     b2 = (uint32_t)(((h & 0xffffffffull) * (uint64_t)nb) >> 32);
+    if (b2 == b1) b2 = (b1 + 1 == nb) ? 0u : b1 + 1;
 }
 
 // Slot of key k in the heavy table, or -1 (two buckets of kBucket keys, read as 128-bit words).
@@ -280,8 +292,16 @@ __device__ __forceinline__ int heavy_find(const uint64_t* lkey, uint32_t nb, uin
 template <typename KT>
 constexpr bool kTypedKey = !std::is_same<KT, void>::value;
 
-template <typename VT, int MODE, typename KT>
-__global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCols kc,
+// The global table is read through a pointer: only the rare atomic path needs it, and kernel
+// arguments kept in scalar registers for the whole loop would spill.
+//
+// FP selects the aggregate states the LDS table keeps: kFieldsSum fixes them at sum + count
+// (SUM, AVG, COUNT; no MIN/MAX) so that the loop carries no per-field tests; kFieldsAny reads
+// them from `fields`.
+constexpr int kFieldsAny = 0, kFieldsSum = 1;
+
+template <typename VT, int MODE, typename KT, int FP>
+__global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec ks, KeyCols kc,
                                                       const VT* __restrict__ vals,
                                                       const uint64_t* __restrict__ vnull,
                                                       const int8_t* __restrict__ mask, PredSet preds,
@@ -290,16 +310,19 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
     extern __shared__ __align__(16) uint64_t lds[];
     constexpr bool HAS_V = VTraits<VT>::has;
     constexpr bool REAL = VTraits<VT>::real;
+    const bool f_sum = FP == kFieldsSum ? true : (fields & F_SUM) != 0;
+    const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
+    const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
     const int H = a.H, T = H + 2, bd = blockDim.x, tid = threadIdx.x;
     const uint32_t nb = (uint32_t)(H / kBucket);
     uint64_t* lkey = lds;
     uint64_t* p = lkey + H;
-    uint64_t* lsum = (fields & F_SUM) ? p : nullptr;
-    p += (fields & F_SUM) ? T : 0;
-    uint64_t* lmin = (fields & F_MIN) ? p : nullptr;
-    p += (fields & F_MIN) ? T : 0;
-    uint64_t* lmax = (fields & F_MAX) ? p : nullptr;
-    p += (fields & F_MAX) ? T : 0;
+    uint64_t* lsum = f_sum ? p : nullptr;
+    p += f_sum ? T : 0;
+    uint64_t* lmin = f_min ? p : nullptr;
+    p += f_min ? T : 0;
+    uint64_t* lmax = f_max ? p : nullptr;
+    p += f_max ? T : 0;
     uint32_t* lcnt = (uint32_t*)p;
     uint32_t* lrows = lcnt + T;
     int32_t* slot_of = (int32_t*)(lrows + T);  // [kmax]
@@ -319,29 +342,53 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
     for (int q = tid; q < a.P; q += bd) cursor[q] = a.accumulate ? a.r_count[region0 + q] : 0u;
     if (tid == 0) nnew[0] = 0;
     __syncthreads();
-    // place each heavy key in the emptier of its two buckets; a key that finds both full
-    // stays cold (it is then aggregated through its partition like any other key)
+    // Place each heavy key in the emptier of its two buckets, the most frequent keys first
+    // (one round per power of two of the sample count), so that the keys carrying most rows
+    // always find a slot.  A key that finds both buckets full stays cold: its rows then go
+    // through its partition like any other key's.
     const int32_t K = min(a.meta[0], a.kmax);
-    for (int k = tid; k < K; k += bd) {
-        const uint64_t key = a.heavy[k];
-        uint32_t b1, b2;
-        buckets_of(hash64(key), nb, b1, b2);
-        int placed = -1;
-        for (int attempt = 0; attempt < 2 * kBucket && placed < 0; attempt++) {
-            int f1 = 0, f2 = 0;
-            for (int j = 0; j < kBucket; j++) {
-                f1 += lkey[b1 * kBucket + j] != kEmpty;
-                f2 += lkey[b2 * kBucket + j] != kEmpty;
-            }
-            const uint32_t bb = (f1 <= f2) ? b1 : b2;
-            if (((f1 <= f2) ? f1 : f2) >= kBucket) break;
-            for (int j = 0; j < kBucket && placed < 0; j++) {
-                unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
-                                                   (unsigned long long)kEmpty, (unsigned long long)key);
-                if (old == kEmpty) placed = (int)(bb * kBucket + j);
-            }
+    // kmax <= 4 * blockDim, so each thread holds at most kPlace heavy keys
+    constexpr int kPlace = 4;
+    static_assert(kPlace * kBlock >= 4096, "heavy keys per thread");
+    uint64_t pkey[kPlace];
+    int pcls[kPlace];
+#pragma unroll
+    for (int i = 0; i < kPlace; i++) {
+        const int k = tid + i * bd;
+        pcls[i] = -1;
+        pkey[i] = 0;
+        if (k < K) {
+            pkey[i] = a.heavy[k];
+            pcls[i] = 31 - __builtin_clz(max(a.heavy_cnt[k], 1u));
         }
-        slot_of[k] = placed;
+    }
+    for (int cls = 31; cls >= 0; cls--) {
+        bool mine = false;
+#pragma unroll
+        for (int i = 0; i < kPlace; i++) mine = mine || pcls[i] == cls;
+        if (!__syncthreads_or(mine)) continue;
+#pragma unroll
+        for (int i = 0; i < kPlace; i++) {
+            if (pcls[i] != cls) continue;
+            uint32_t b1, b2;
+            buckets_of(hash64(pkey[i]), nb, b1, b2);
+            int placed = -1;
+            for (int attempt = 0; attempt < 2 * kBucket && placed < 0; attempt++) {
+                int f1 = 0, f2 = 0;
+                for (int j = 0; j < kBucket; j++) {
+                    f1 += lkey[b1 * kBucket + j] != kEmpty;
+                    f2 += lkey[b2 * kBucket + j] != kEmpty;
+                }
+                const uint32_t bb = (f1 <= f2) ? b1 : b2;
+                if (((f1 <= f2) ? f1 : f2) >= kBucket) break;
+                for (int j = 0; j < kBucket && placed < 0; j++) {
+                    unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
+                                                       (unsigned long long)kEmpty, (unsigned long long)pkey[i]);
+                    if (old == kEmpty) placed = (int)(bb * kBucket + j);
+                }
+            }
+            slot_of[tid + i * bd] = placed;
+        }
     }
     __syncthreads();
 
@@ -416,7 +463,8 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
                 continue;
             }
             if (!vn) {
-                const uint32_t q = part_of(g, h);
+                // part_shift is 56..64; the split shift gives 0 for 64 without a select
+                const uint32_t q = (uint32_t)((h >> 1) >> (a.part_shift - 1));
                 const uint32_t c = atomicAdd(&cursor[q], 1u);
                 if (c < (uint32_t)a.cap) {
                     const int64_t at = (region0 + q) * (int64_t)a.cap + c;
@@ -426,6 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
                 }
             }
             // NULL value or full region: atomic path
+            const GView& g = *gp;
             bool is_new = false;
             const int64_t gs = g_slot1(g, kk, false, h, &is_new);
             if (is_new) atomicAdd(nnew, 1u);
@@ -463,7 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(GView g, KeySpec ks, KeyCo
         if (lmax) a.p_max[at] = mx;
     }
     for (int q = tid; q < a.P; q += bd) a.r_count[region0 + q] = min(cursor[q], (uint32_t)a.cap);
-    if (tid == 0) g_count_new(g, nnew[0]);
+    if (tid == 0) g_count_new(*gp, nnew[0]);
 }
 
 // ---- phase 2 -------------------------------------------------------------------------------
@@ -487,7 +536,8 @@ __host__ __device__ inline size_t p2_lds_bytes(int32_t H2, int32_t W, uint32_t f
     size_t b = (size_t)H2 * 8;
     b += T * 8 * (((fields & F_SUM) ? 1 : 0) + ((fields & F_MIN) ? 1 : 0) + ((fields & F_MAX) ? 1 : 0));
     b += T * 4 * 2;
-    b += ((size_t)W + 1) * 4 + 16;
+    b += 16;
+    (void)W;
     return b;
 }
 
@@ -498,7 +548,6 @@ __device__ __forceinline__ uint32_t p2_home(uint64_t h, int32_t H2) {
 template <typename VT>
 __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, P2Args a) {
     extern __shared__ __align__(16) uint64_t lds[];
-    __shared__ int32_t wsum[16];
     constexpr bool HAS_V = VTraits<VT>::has;
     constexpr bool REAL = VTraits<VT>::real;
     const int H = a.H2, T = H + 2, bd = blockDim.x, tid = threadIdx.x, part = blockIdx.x;
@@ -510,10 +559,9 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     p += (fields & F_MIN) ? T : 0;
     uint64_t* tmax = (fields & F_MAX) ? p : nullptr;
     p += (fields & F_MAX) ? T : 0;
-    uint32_t* tcnt = (uint32_t*)p;
-    uint32_t* trows = tcnt + T;
-    int32_t* roff = (int32_t*)(trows + T);
-    uint32_t* misc = (uint32_t*)(roff + a.W + 1);  // [0] used slots, [1] new keys
+    uint32_t* tcnt = (uint32_t*)p;    // non-NULL values
+    uint32_t* trows = tcnt + T;       // rows beyond tcnt (rows with a NULL value)
+    uint32_t* misc = trows + T;       // [0] used slots, [1] new keys, [2] overflow rows, [3] rows
 
     auto clear_slot = [&](int s) {
         if (s < H) tkey[s] = kEmpty;
@@ -527,37 +575,53 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     if (tid == 0) {
         misc[0] = 0;
         misc[1] = 0;
-    }
-    // prefix of this partition's region sizes over the phase-1 workgroups (W <= blockDim)
-    {
-        const int32_t c = tid < a.W ? (int32_t)a.r_count[(int64_t)tid * a.P + part] : 0;
-        int32_t total;
-        const int32_t ex = block_excl_scan(c, wsum, &total);
-        if (tid < a.W) roff[tid] = ex;
-        if (tid == 0) roff[a.W] = total;
+        misc[2] = 0;
+        misc[3] = 0;
     }
     __syncthreads();
 
-    // The chunking below keeps the table at most 4/5 full, so a free slot always exists;
-    // the bound only guards against a broken invariant (reported as a full table).
+    // A new key takes a slot only while fewer than `limit` slots are in use (reserved before
+    // the claim, returned if the key turns out to be there already); past that the caller
+    // aggregates the row straight into the global table.  Linear probing without deletion:
+    // once an empty slot is seen, the key is not further along the chain.
+    const uint32_t limit = (uint32_t)((int64_t)H * 4 / 5);
     auto insert = [&](uint64_t key) -> int {
         uint32_t s = p2_home(hash64(key), H);
+        bool reserved = false;
         for (int it = 0; it < H; it++) {
             const uint64_t cur = tkey[s];
-            if (cur == key) return (int)s;
+            if (cur == key) {
+                if (reserved) atomicSub(&misc[0], 1u);
+                return (int)s;
+            }
             if (cur == kEmpty) {
-                unsigned long long old = atomicCAS((unsigned long long*)&tkey[s], (unsigned long long)kEmpty,
-                                                   (unsigned long long)key);
-                if (old == kEmpty) {
-                    atomicAdd(&misc[0], 1u);
+                if (!reserved) {
+                    if (atomicAdd(&misc[0], 1u) >= limit) {
+                        atomicSub(&misc[0], 1u);
+                        return -1;
+                    }
+                    reserved = true;
+                }
+                const unsigned long long old = atomicCAS((unsigned long long*)&tkey[s],
+                                                         (unsigned long long)kEmpty, (unsigned long long)key);
+                if (old == kEmpty) return (int)s;
+                if (old == key) {
+                    atomicSub(&misc[0], 1u);
                     return (int)s;
                 }
-                if (old == key) return (int)s;
             }
             s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
         }
-        g_set_error(g, QK_HASH_TABLE_FULL);
+        if (reserved) atomicSub(&misc[0], 1u);
         return -1;
+    };
+    // rows of keys that found no LDS slot
+    auto add_global = [&](uint64_t key, const Part& pr) {
+        bool is_new = false;
+        const int64_t gs = g_slot1(g, key, false, hash64(key), &is_new);
+        if (is_new) atomicAdd(&misc[1], 1u);
+        if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+        atomicAdd(&misc[2], 1u);
     };
     auto add = [&](int s, uint32_t rows, uint32_t cnt, uint64_t sum, uint64_t mn, uint64_t mx) {
         if (rows) atomicAdd(&trows[s], rows);
@@ -576,7 +640,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     auto part_of_slot = [&](int s) {
         Part q;
         q.cnt = tcnt[s];
-        q.cntall = trows[s];
+        q.cntall = trows[s] + tcnt[s];
         q.sum = tsum ? tsum[s] : 0;
         q.mn = tmin ? tmin[s] : ~0ull;
         q.mx = tmax ? tmax[s] : 0;
@@ -584,7 +648,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     };
     // fold the LDS table into this partition's sub-table (this workgroup is its only writer)
     auto merge = [&]() {
+        // rows that went straight to the global table (atomics at device scope) are complete
+        // and visible to this workgroup's plain reads below
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         for (int s = tid; s < H; s += bd) {
             const uint64_t key = tkey[s];
             if (key == kEmpty) continue;
@@ -626,9 +694,19 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                     if (tmax) mx = max(mx, (uint64_t)a.p_max[at]);
                 }
                 if (!rows) continue;
+                const uint64_t hsum = REAL ? (uint64_t)__double_as_longlong(dsum) : isum;
                 const int s = insert(a.heavy[hk0 + ki]);
-                if (s < 0) continue;
-                add(s, rows, cnt, REAL ? (uint64_t)__double_as_longlong(dsum) : isum, mn, mx);
+                if (s >= 0) {
+                    add(s, rows - cnt, cnt, hsum, mn, mx);
+                } else {
+                    Part q;
+                    q.cnt = cnt;
+                    q.cntall = rows;
+                    q.sum = hsum;
+                    q.mn = mn;
+                    q.mx = mx;
+                    add_global(a.heavy[hk0 + ki], q);
+                }
             }
         }
     }
@@ -638,63 +716,61 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
             const int64_t at = (int64_t)w * stride_k + a.kmax + side;
             const uint32_t rows = a.p_rows[at];
             if (!rows) continue;
-            add(H + side, rows, a.p_cnt[at], tsum ? a.p_sum[at] : 0, tmin ? a.p_min[at] : ~0ull,
-                tmax ? a.p_max[at] : 0);
+            add(H + side, rows - a.p_cnt[at], a.p_cnt[at], tsum ? a.p_sum[at] : 0,
+                tmin ? a.p_min[at] : ~0ull, tmax ? a.p_max[at] : 0);
         }
     }
 
-    // cold rows of this partition, in passes that cannot overfill the LDS table: a pass takes
-    // at most as many rows as the table has room for new keys (merging first when that room
-    // is small), and a thread loads all of its rows of a pass before it inserts any
-    const int64_t total = roff[a.W];
-    const int64_t limit = (int64_t)H * 4 / 5;
-    const int64_t step = std::min<int64_t>((int64_t)bd * kP2Rows, limit);
-    for (int64_t base = 0; base < total;) {
+    // Cold rows of this partition: in each round every wave takes the partition's region of
+    // one phase-1 workgroup and walks it 64 x kP2Rows rows at a time, loading them all before
+    // inserting any.  A round in which many rows found no LDS slot means the partition holds
+    // more keys than the table: the table is then merged and cleared before the next round.
+    const int lane = tid & 63, wave = tid >> 6, nwaves = bd >> 6;
+    for (int r0 = 0; r0 < a.W; r0 += nwaves) {
         __syncthreads();
-        // every thread reads the fill level before any thread inserts again
-        int64_t room = limit - (int64_t)misc[0];
+        const bool flush = misc[2] > 0 && (uint64_t)misc[2] * 8 > misc[3];
         __syncthreads();
-        if (room < std::min<int64_t>(bd, limit)) {
-            merge();
-            room = limit;
+        if (flush) merge();
+        if (tid == 0) {
+            misc[2] = 0;
+            misc[3] = 0;
         }
-        const int64_t end = base + std::min<int64_t>(std::min<int64_t>(room, step), total - base);
-        uint64_t key[kP2Rows];
-        typename std::conditional<HAS_V, VT, int>::type val[kP2Rows];
+        __syncthreads();
+        const int w = r0 + wave;
+        if (w >= a.W) continue;
+        const int64_t region = ((int64_t)w * a.P + part) * (int64_t)a.cap;
+        const int32_t cnt = (int32_t)a.r_count[(int64_t)w * a.P + part];
+        if (lane == 0) atomicAdd(&misc[3], (uint32_t)cnt);
+        for (int32_t j0 = 0; j0 < cnt; j0 += 64 * kP2Rows) {
+            uint64_t key[kP2Rows];
+            typename std::conditional<HAS_V, VT, int>::type val[kP2Rows];
 #pragma unroll
-        for (int i = 0; i < kP2Rows; i++) {
-            const int64_t j = base + (int64_t)i * bd + tid;
-            key[i] = kEmpty;
-            val[i] = {};
-            if (j >= end) continue;
-            // region of row j: last w with roff[w] <= j
-            int lo = 0, hi = a.W;  // roff[lo] <= j < roff[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if ((int64_t)roff[mid] <= j) lo = mid; else hi = mid;
+            for (int i = 0; i < kP2Rows; i++) {
+                const int32_t j = min(j0 + i * 64 + lane, cnt - 1);
+                key[i] = a.r_keys[region + j];
+                if constexpr (HAS_V) val[i] = ((const VT*)a.r_vals)[region + j]; else val[i] = 0;
             }
-            const int64_t at = ((int64_t)lo * a.P + part) * (int64_t)a.cap + (j - roff[lo]);
-            key[i] = a.r_keys[at];
-            if constexpr (HAS_V) val[i] = ((const VT*)a.r_vals)[at];
-        }
 #pragma unroll
-        for (int i = 0; i < kP2Rows; i++) {
-            if (base + (int64_t)i * bd + tid >= end) continue;
-            const int s = insert(key[i]);
-            if (s < 0) continue;
-            if constexpr (HAS_V) {
-                const Part pr = row_part<VT>(val[i], false);
-                add(s, 1, 1, pr.sum, pr.mn, pr.mx);
-            } else {
-                add(s, 1, 1, 0, ~0ull, 0);
+            for (int i = 0; i < kP2Rows; i++) {
+                if (j0 + i * 64 + lane >= cnt) continue;
+                Part pr;
+                if constexpr (HAS_V) {
+                    pr = row_part<VT>(val[i], false);
+                } else {
+                    pr = row_part<NoValue>(NoValue{}, false);
+                }
+                const int s = insert(key[i]);
+                if (s >= 0)
+                    add(s, 0, 1, pr.sum, pr.mn, pr.mx);
+                else
+                    add_global(key[i], pr);
             }
         }
-        base = end;
     }
     merge();
     if (part == 0 && tid < 2) {
         const int s = H + tid;
-        if (trows[s]) {
+        if (trows[s] + tcnt[s]) {  // rows reached the side slot
             const uint64_t gs = g.cap + tid;
             if (g_claim_side(g, gs) && tid == 1) atomicAdd(&misc[1], 1u);
             g_accum_owned(g, (int64_t)gs, part_of_slot(s), fields);
@@ -729,17 +805,22 @@ int launch_sample_m(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols&
     switch (mode) {
     case 0: return launch_sample<VT, 0>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
     case 1: return launch_sample<VT, 1>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
-    default: return launch_sample<VT, 2>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+    case 2: return launch_sample<VT, 2>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+    default:
+        if constexpr (VTraits<VT>::has)
+            return launch_sample<VT, 3>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+        return fail(QK_ERROR, "value range filter without a value column");
     }
 }
 
-template <typename VT, int MODE, typename KT>
+template <typename VT, int MODE, typename KT, int FP>
 int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc, const void* vals,
               const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
               int64_t n, uint32_t fields, hipStream_t s) {
     // gfx950 launches up to 160 KB of LDS per workgroup without a function attribute
     P1Args a;
     a.heavy = b.heavy_keys;
+    a.heavy_cnt = b.heavy_cnt;
     a.meta = b.heavy_meta;
     a.kmax = b.kmax;
     a.H = b.H1;
@@ -754,22 +835,27 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
     a.r_vals = b.r_vals;
     a.r_count = b.r_count;
     a.zero = b.zero_word;
+    a.part_shift = g.part_shift;
     a.accumulate = b.pending_rows > 0 ? 1 : 0;
     const size_t lds = p1_lds_bytes(b.H1, b.P, b.kmax, fields);
-    k_tp_phase1<VT, MODE, KT><<<b.W, b.block, lds, s>>>(g, ks, kc, (const VT*)vals, vnull, mask,
-                                                        ps, vpred, n, fields, a);
+    k_tp_phase1<VT, MODE, KT, FP><<<b.W, b.block, lds, s>>>(b.gview, ks, kc, (const VT*)vals, vnull,
+                                                            mask, ps, vpred, n, fields, a);
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }
 
-template <typename VT, typename KT>
+template <typename VT, typename KT, int FP>
 int launch_p1_m(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
                 const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
                 uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
     switch (mode) {
-    case 0: return launch_p1<VT, 0, KT>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    case 1: return launch_p1<VT, 1, KT>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    default: return launch_p1<VT, 2, KT>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 0: return launch_p1<VT, 0, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 1: return launch_p1<VT, 1, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 2: return launch_p1<VT, 2, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    default:
+        if constexpr (VTraits<VT>::has)
+            return launch_p1<VT, 3, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        return fail(QK_ERROR, "value range filter without a value column");
     }
 }
 
@@ -779,11 +865,18 @@ template <typename VT>
 int launch_p1_k(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
                 const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
                 uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
-    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_LONG)
-        return launch_p1_m<VT, int64_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_INT)
-        return launch_p1_m<VT, int32_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    return launch_p1_m<VT, void>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    const bool sum_only = (fields & F_SUM) && !(fields & (F_MIN | F_MAX));
+    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_LONG) {
+        if (sum_only)
+            return launch_p1_m<VT, int64_t, kFieldsSum>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        return launch_p1_m<VT, int64_t, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    }
+    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_INT) {
+        if (sum_only)
+            return launch_p1_m<VT, int32_t, kFieldsSum>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        return launch_p1_m<VT, int32_t, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    }
+    return launch_p1_m<VT, void, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
 }
 
 template <typename VT>
@@ -817,9 +910,9 @@ bool twophase_supported(int32_t kw, int32_t value_type) {
 }
 
 void twophase_free(TwoPhaseBufs& b) {
-    void* ptrs[] = {b.heavy_keys, b.heavy_meta, b.sample_keys, b.sample_cnt, b.hist, b.cand,
+    void* ptrs[] = {b.heavy_keys, b.heavy_cnt, b.cand_cnt, b.heavy_meta, b.sample_keys, b.sample_cnt, b.hist, b.cand,
                     b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_keys, b.r_vals, b.r_count,
-                    b.zero_word};
+                    b.zero_word, b.gview};
     for (void* p : ptrs)
         if (p) pool_free(p);
     TwoPhaseBufs fresh;
@@ -885,6 +978,8 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
         if (st == QK_SUCCESS) st = pool_alloc(ptr, bytes);
     };
     A((void**)&b.heavy_keys, (size_t)b.kmax * 8);
+    A((void**)&b.heavy_cnt, (size_t)b.kmax * 4);
+    A((void**)&b.cand_cnt, (size_t)b.kmax * 4);
     A((void**)&b.heavy_meta, (size_t)(parts + 8) * 4);
     A((void**)&b.sample_keys, (size_t)sslots * 8);
     A((void**)&b.sample_cnt, (size_t)sslots * 4);
@@ -899,6 +994,7 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     if (vbytes) A((void**)&b.r_vals, (size_t)regions * vbytes);
     A((void**)&b.r_count, (size_t)b.W * parts * 4);
     A((void**)&b.zero_word, 64);
+    A((void**)&b.gview, sizeof(GView));
     if (st != QK_SUCCESS) {
         twophase_free(b);
         return st;
@@ -935,8 +1031,9 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
     k_tp_hist<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots, b.hist);
     k_tp_threshold<<<1, kHistBins, 0, s>>>(b.hist, b.kmax, b.heavy_meta);
     k_tp_collect<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots,
-                                                               b.kmax, b.heavy_meta, b.cand);
-    k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.kmax, b.P, g.part_shift, b.heavy_meta, b.heavy_keys);
+                                                               b.kmax, b.heavy_meta, b.cand, b.cand_cnt);
+    k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.cand_cnt, b.kmax, b.P, g.part_shift, b.heavy_meta, b.heavy_keys,
+                                 b.heavy_cnt);
     QK_HIP(hipGetLastError());
     b.heavy_ready = true;
     return QK_SUCCESS;
@@ -951,6 +1048,13 @@ int twophase_phase1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
         return fail(QK_ERROR, "group by: value column type changed between blocks");
     if (b.pending_rows + n > b.rows_alloc || n > b.block_alloc)
         return fail(QK_ERROR, "group by: two-phase buffers too small for the block");
+    if (g.part_shift < 33 || g.part_shift > 64)
+        return fail(QK_ERROR, "group by: unsupported partition count for the two-phase path");
+    if (!b.gview_valid || memcmp(&b.gview_host, &g, sizeof(GView)) != 0) {
+        b.gview_host = g;
+        QK_HIP(hipMemcpyAsync(b.gview, &b.gview_host, sizeof(GView), hipMemcpyHostToDevice, s));
+        b.gview_valid = true;
+    }
     if (ev0) QK_HIP(hipEventRecord(ev0, s));
     int st;
     if (!vals) {

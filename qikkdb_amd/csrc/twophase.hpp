@@ -35,11 +35,13 @@ struct TwoPhaseBufs {
     int32_t pend_vtype = 0;   // value type of the pending rows (0 = no value column)
     // heavy set
     uint64_t* heavy_keys = nullptr;   // [kmax], sorted by partition
+    uint32_t* heavy_cnt = nullptr;    // [kmax] sample count of each heavy key
     int32_t* heavy_meta = nullptr;    // [0] = K, [1] = threshold, [2..] = start per partition (P+1)
     uint64_t* sample_keys = nullptr;  // [2^(sample_bits+1)]
     uint32_t* sample_cnt = nullptr;
     uint32_t* hist = nullptr;         // [kHistBins]
     uint64_t* cand = nullptr;         // [kmax] unsorted heavy candidates
+    uint32_t* cand_cnt = nullptr;     // [kmax] their sample counts
     // per phase-1 workgroup partials [W][kmax + 2]
     uint32_t* p_cnt = nullptr;
     uint32_t* p_rows = nullptr;
@@ -51,6 +53,9 @@ struct TwoPhaseBufs {
     void* r_vals = nullptr;
     uint32_t* r_count = nullptr;      // [W][P]
     uint64_t* zero_word = nullptr;    // 64 zero bytes
+    GView* gview = nullptr;           // device copy of the table view (phase 1 reads it there)
+    GView gview_host{};               // what gview holds
+    bool gview_valid = false;
     bool heavy_ready = false;
 };
 

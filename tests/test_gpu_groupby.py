@@ -367,3 +367,68 @@ def test_two_phase_reset_drops_pending_rows():
     got = gb.results_host()
     gb.close()
     compare(got, cpu_group_by(aggs, [kb], vb), 2)
+
+
+# WHERE clauses on the value column alone are folded into one interval per comparison
+# domain; each case crosses a conversion boundary of the usual arithmetic conversions
+FOLD_CASES = [
+    # (value dtype, [(op, const, const type), ...])
+    (np.int32, [(qk.GREATER, 100, qk.INT), (qk.LESS_EQUAL, 5000, qk.LONG)]),
+    (np.int32, [(qk.GREATER_EQUAL, 16777217, qk.FLOAT), (qk.LESS, 2.5e7, qk.DOUBLE)]),
+    (np.int32, [(qk.EQUAL, 777, qk.INT)]),
+    (np.int64, [(qk.GREATER, 2**53 + 1, qk.DOUBLE), (qk.LESS, 2**62, qk.LONG)]),
+    (np.int64, [(qk.GREATER, 3.0e15, qk.FLOAT)]),
+    (np.int64, [(qk.GREATER, 10, qk.INT), (qk.LESS, 5, qk.INT)]),          # empty
+    (np.float32, [(qk.GREATER, 0.1, qk.DOUBLE), (qk.LESS_EQUAL, 0.1, qk.FLOAT)]),
+    (np.float32, [(qk.GREATER_EQUAL, 0.5, qk.FLOAT), (qk.LESS_EQUAL, 4.0, qk.FLOAT)]),
+    (np.float32, [(qk.GREATER, 0, qk.INT), (qk.GREATER, -1, qk.LONG)]),
+    (np.float32, [(qk.EQUAL, 0.0, qk.FLOAT)]),
+    (np.float64, [(qk.GREATER, 2**60 + 1, qk.LONG), (qk.LESS_EQUAL, 1e300, qk.DOUBLE)]),
+    (np.float64, [(qk.GREATER_EQUAL, 1.0, qk.FLOAT), (qk.GREATER, 1.0, qk.DOUBLE)]),
+]
+
+
+def fold_values(vt, n, rng):
+    if vt == np.int32:
+        v = rng.integers(-3000, 30_000_000, n).astype(np.int64)
+        v[::7] = 16777216 + rng.integers(-3, 4, v[::7].size)
+        v[::11] = 777
+        return v.astype(np.int32)
+    if vt == np.int64:
+        v = rng.integers(-2**62, 2**62, n)
+        v[::5] = 2**53 + rng.integers(-3, 4, v[::5].size)
+        v[::9] = 3_000_000_000_000_000 + rng.integers(-2**20, 2**20, v[::9].size)
+        v[::13] = rng.integers(0, 20, v[::13].size)
+        return v.astype(np.int64)
+    if vt == np.float32:
+        v = (rng.random(n) * 8 - 2).astype(np.float32)
+        v[::5] = np.float32(0.1)
+        v[::7] = np.nextafter(np.float32(0.1), np.float32(1))
+        v[::11] = 0.0
+        v[::13] = -0.0
+        v[::17] = np.nan
+        v[::19] = np.float32(4.0)
+        return v
+    v = rng.random(n) * 4 - 1
+    v[::5] = float(2**60)
+    v[::7] = np.nextafter(float(2**60), 2.0**61)
+    v[::11] = 1.0
+    v[::13] = np.nan
+    v[::17] = np.inf
+    return v.astype(np.float64)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("case", range(len(FOLD_CASES)))
+def test_value_range_fold_matches_predicates(case, path):
+    vt, preds = FOLD_CASES[case]
+    rng = np.random.default_rng(500 + case)
+    n = 300_000
+    keys = rng.integers(0, 5000, n).astype(np.int64)
+    vals = fold_values(vt, n, rng)
+    vn = rng.random(n) < 0.05
+    w = [Where(op, "value", c, ct) for op, c, ct in preds]
+    aggs = [qk.AGG_COUNT, qk.AGG_COUNT_ALL, qk.AGG_MIN, qk.AGG_MAX]
+    gpu = gpu_group_by(aggs, [keys], vals, None, vn, where=w, capacity=10_000, path=path)
+    cpu = cpu_group_by(aggs, [keys], vals, None, vn, where=w)
+    compare(gpu, cpu, len(aggs))
