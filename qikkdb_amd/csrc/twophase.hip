@@ -323,16 +323,16 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     p += f_min ? T : 0;
     uint64_t* lmax = f_max ? p : nullptr;
     p += f_max ? T : 0;
-    uint32_t* lcnt = (uint32_t*)p;
-    uint32_t* lrows = lcnt + T;
-    int32_t* slot_of = (int32_t*)(lrows + T);  // [kmax]
+    uint32_t* lcnt = (uint32_t*)p;   // rows with a non-NULL value
+    uint32_t* lnul = lcnt + T;       // rows with a NULL value
+    int32_t* slot_of = (int32_t*)(lnul + T);  // [kmax]
     uint32_t* cursor = (uint32_t*)(slot_of + a.kmax);
     uint32_t* nnew = cursor + a.P;
 
     for (int s = tid; s < T; s += bd) {
         if (s < H) lkey[s] = kEmpty;
         lcnt[s] = 0;
-        lrows[s] = 0;
+        lnul[s] = 0;
         if (lsum) lsum[s] = 0;
         if (lmin) lmin[s] = ~0ull;
         if (lmax) lmax[s] = 0;
@@ -405,50 +405,75 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     const int64_t vstep = vnull ? 1 : 0;
     const uint64_t* kbits = kc.nm[0] ? kc.nm[0] : a.zero;
     const int64_t kstep = kc.nm[0] ? 1 : 0;
-    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    using VV = typename std::conditional<HAS_V, VT, int>::type;
+    struct Tile {
         uint64_t key[kP1Steps], vword[kP1Steps], kword[kP1Steps];
         bool kn[kP1Steps];
-        typename std::conditional<HAS_V, VT, int>::type val[kP1Steps];
+        VV val[kP1Steps];
+    };
+    auto load_tile = [&](int64_t tile, Tile& t) {
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
             const int64_t r0 = min(tile * tile_rows + (int64_t)k * bd + wave_row, n - 1);
             const int64_t r = min(r0 + lane, n - 1);
-            vword[k] = vbits[(r0 >> 6) * vstep];
-            kword[k] = 0;
-            kn[k] = false;
+            t.vword[k] = vbits[(r0 >> 6) * vstep];
+            t.kword[k] = 0;
+            t.kn[k] = false;
             if constexpr (kTypedKey<KT>) {
-                kword[k] = kbits[(r0 >> 6) * kstep];
-                key[k] = (uint64_t)(int64_t)((const KT*)kc.col[0])[r];
+                t.kword[k] = kbits[(r0 >> 6) * kstep];
+                t.key[k] = (uint64_t)(int64_t)((const KT*)kc.col[0])[r];
             } else {
                 uint64_t kk[1];
-                load_key<1>(ks, kc, r, kk, kn[k]);
-                key[k] = kk[0];
+                load_key<1>(ks, kc, r, kk, t.kn[k]);
+                t.key[k] = kk[0];
             }
-            if constexpr (HAS_V) val[k] = vals[r]; else val[k] = 0;
+            if constexpr (HAS_V) t.val[k] = vals[r]; else t.val[k] = 0;
         }
+    };
+    // A tile's rows after the WHERE: key (0 for a NULL key), value, and per step whether the
+    // row passes and whether its key / value is NULL.
+    struct Rows {
+        uint64_t key[kP1Steps];
+        VV val[kP1Steps];
+        uint32_t pass, vnull, knull;
+    };
+    auto filter_tile = [&](int64_t tile, const Tile& t, Rows& o) {
+        o.pass = o.vnull = o.knull = 0;
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
             const int64_t r = tile * tile_rows + (int64_t)k * bd + tid;
-            if (r >= n) continue;
-            const bool vn = (vword[k] >> lane) & 1;
+            const bool vn = (t.vword[k] >> lane) & 1;
+            bool kn = t.kn[k];
+            uint64_t kk = t.key[k];
             if constexpr (kTypedKey<KT>) {
-                kn[k] = (kword[k] >> lane) & 1;
-                if (kn[k]) key[k] = 0;
+                kn = (t.kword[k] >> lane) & 1;
+                if (kn) kk = 0;
             }
-            if (!row_passes<VT, MODE>(preds, vpred, mask, r, val[k], vn)) continue;
-            const uint64_t kk = key[k];
+            o.key[k] = kk;
+            o.val[k] = t.val[k];
+            const bool pass = r < n && row_passes<VT, MODE>(preds, vpred, mask, r, t.val[k], vn);
+            o.pass |= (uint32_t)pass << k;
+            o.vnull |= (uint32_t)vn << k;
+            o.knull |= (uint32_t)kn << k;
+        }
+    };
+    auto aggregate_rows = [&](const Rows& o) {
+#pragma unroll
+        for (int k = 0; k < kP1Steps; k++) {
+            if (!((o.pass >> k) & 1)) continue;
+            const bool vn = (o.vnull >> k) & 1, kn = (o.knull >> k) & 1;
+            const uint64_t kk = o.key[k];
             const uint64_t h = hash64(kk);
             int ls;
-            if (kn[k])
+            if (kn)
                 ls = H;
             else if (kk == kEmpty)
                 ls = H + 1;
             else
                 ls = heavy_find(lkey, nb, kk, h);
             Part pr;
-            if constexpr (HAS_V) pr = row_part<VT>(val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
+            if constexpr (HAS_V) pr = row_part<VT>(o.val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
             if (ls >= 0) {
-                atomicAdd(&lrows[ls], 1u);
                 if (pr.cnt) {
                     atomicAdd(&lcnt[ls], 1u);
                     if (lsum) {
@@ -459,6 +484,8 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                     }
                     if (lmin) atomicMin((unsigned long long*)&lmin[ls], (unsigned long long)pr.mn);
                     if (lmax) atomicMax((unsigned long long*)&lmax[ls], (unsigned long long)pr.mx);
+                } else {
+                    atomicAdd(&lnul[ls], 1u);
                 }
                 continue;
             }
@@ -469,7 +496,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                 if (c < (uint32_t)a.cap) {
                     const int64_t at = (region0 + q) * (int64_t)a.cap + c;
                     a.r_keys[at] = kk;
-                    if constexpr (HAS_V) ((VT*)a.r_vals)[at] = val[k];
+                    if constexpr (HAS_V) ((VT*)a.r_vals)[at] = o.val[k];
                     continue;
                 }
             }
@@ -480,6 +507,13 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             if (is_new) atomicAdd(nnew, 1u);
             if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
         }
+    };
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        Tile t;
+        Rows o;
+        load_tile(tile, t);
+        filter_tile(tile, t, o);
+        aggregate_rows(o);
     }
 
     __syncthreads();
@@ -489,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     for (int k = tid; k < K + 2; k += bd) {
         const int s = k < K ? slot_of[k] : H + (k - K);
         const int64_t at = pbase + (k < K ? k : a.kmax + (k - K));
-        uint32_t rows = s >= 0 ? lrows[s] : 0u, cnt = s >= 0 ? lcnt[s] : 0u;
+        uint32_t cnt = s >= 0 ? lcnt[s] : 0u, rows = s >= 0 ? lcnt[s] + lnul[s] : 0u;
         uint64_t sum = (lsum && s >= 0) ? lsum[s] : 0ull;
         uint64_t mn = (lmin && s >= 0) ? lmin[s] : ~0ull, mx = (lmax && s >= 0) ? lmax[s] : 0ull;
         if (a.accumulate) {
@@ -545,7 +579,7 @@ __device__ __forceinline__ uint32_t p2_home(uint64_t h, int32_t H2) {
     return (uint32_t)((((h >> 8) & 0xffffffffull) * (uint64_t)(uint32_t)H2) >> 32);
 }
 
-template <typename VT>
+template <typename VT, int FP>
 __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, P2Args a) {
     extern __shared__ __align__(16) uint64_t lds[];
     constexpr bool HAS_V = VTraits<VT>::has;
@@ -553,12 +587,15 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     const int H = a.H2, T = H + 2, bd = blockDim.x, tid = threadIdx.x, part = blockIdx.x;
     uint64_t* tkey = lds;
     uint64_t* p = tkey + H;
-    uint64_t* tsum = (fields & F_SUM) ? p : nullptr;
-    p += (fields & F_SUM) ? T : 0;
-    uint64_t* tmin = (fields & F_MIN) ? p : nullptr;
-    p += (fields & F_MIN) ? T : 0;
-    uint64_t* tmax = (fields & F_MAX) ? p : nullptr;
-    p += (fields & F_MAX) ? T : 0;
+    const bool f_sum = FP == kFieldsSum ? true : (fields & F_SUM) != 0;
+    const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
+    const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
+    uint64_t* tsum = f_sum ? p : nullptr;
+    p += f_sum ? T : 0;
+    uint64_t* tmin = f_min ? p : nullptr;
+    p += f_min ? T : 0;
+    uint64_t* tmax = f_max ? p : nullptr;
+    p += f_max ? T : 0;
     uint32_t* tcnt = (uint32_t*)p;    // non-NULL values
     uint32_t* trows = tcnt + T;       // rows beyond tcnt (rows with a NULL value)
     uint32_t* misc = trows + T;       // [0] used slots, [1] new keys, [2] overflow rows, [3] rows
@@ -585,8 +622,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     // aggregates the row straight into the global table.  Linear probing without deletion:
     // once an empty slot is seen, the key is not further along the chain.
     const uint32_t limit = (uint32_t)((int64_t)H * 4 / 5);
-    auto insert = [&](uint64_t key) -> int {
-        uint32_t s = p2_home(hash64(key), H);
+    auto insert_from = [&](uint64_t key, uint32_t s) -> int {
         bool reserved = false;
         for (int it = 0; it < H; it++) {
             const uint64_t cur = tkey[s];
@@ -615,6 +651,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         if (reserved) atomicSub(&misc[0], 1u);
         return -1;
     };
+    auto insert = [&](uint64_t key) -> int { return insert_from(key, p2_home(hash64(key), H)); };
     // rows of keys that found no LDS slot
     auto add_global = [&](uint64_t key, const Part& pr) {
         bool is_new = false;
@@ -750,6 +787,13 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 key[i] = a.r_keys[region + j];
                 if constexpr (HAS_V) val[i] = ((const VT*)a.r_vals)[region + j]; else val[i] = 0;
             }
+            // first probe of every row at once; most rows find their key there
+            uint32_t home[kP2Rows];
+            uint64_t first[kP2Rows];
+#pragma unroll
+            for (int i = 0; i < kP2Rows; i++) home[i] = p2_home(hash64(key[i]), H);
+#pragma unroll
+            for (int i = 0; i < kP2Rows; i++) first[i] = tkey[home[i]];
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 if (j0 + i * 64 + lane >= cnt) continue;
@@ -759,7 +803,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 } else {
                     pr = row_part<NoValue>(NoValue{}, false);
                 }
-                const int s = insert(key[i]);
+                const int s = first[i] == key[i] ? (int)home[i] : insert_from(key[i], home[i]);
                 if (s >= 0)
                     add(s, 0, 1, pr.sum, pr.mn, pr.mx);
                 else
@@ -897,7 +941,10 @@ int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
     a.r_keys = b.r_keys;
     a.r_vals = b.r_vals;
     a.r_count = b.r_count;
-    k_tp_phase2<VT><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
+    if ((fields & F_SUM) && !(fields & (F_MIN | F_MAX)))
+        k_tp_phase2<VT, kFieldsSum><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
+    else
+        k_tp_phase2<VT, kFieldsAny><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }
