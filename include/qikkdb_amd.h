@@ -299,6 +299,73 @@ int qk_gb_set_defer_rows(qk_groupby h, int64_t rows);
  * out[8] phase-2 LDS slots, out[9] heavy key limit.  n = entries to fill (at most 10). */
 int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
 
+/* ======================================================================================
+ * Batched queries (no reference counterpart: the reference runs one query at a time, each
+ * block through filter -> reconstruct -> GroupBy, GpuSqlDispatcher.cpp:1107-1179)
+ *
+ * Many independent `SELECT key, SUM(v), MIN(v), MAX(v), COUNT(v) FROM t WHERE lo <= ts < hi
+ * GROUP BY key` queries over one device-resident table, run together by one persistent
+ * kernel that takes (query, tile) work units from a device queue.  Each query reads a
+ * contiguous range of the table's blocks and groups by a computed key
+ *     key = (a * b_range + b) mod key_mod        (Euclidean remainder, 0 <= key < key_mod)
+ * of two int32 columns, so its groups are the dense range [0, key_mod).  A query with at
+ * most `lds_groups` groups aggregates each work unit in LDS and adds the unit's groups to
+ * its result with atomics; a larger one aggregates with device atomics directly.
+ * ====================================================================================== */
+
+typedef struct qk_batch_columns {
+    const int32_t* a;       /* key part: key = (a * b_range + b) mod key_mod */
+    const int32_t* b;
+    const int64_t* ts;      /* filtered column */
+    const int64_t* value;   /* aggregated column (no NULLs) */
+    int64_t rows;           /* table rows */
+    int32_t block_rows;     /* rows per block */
+    int32_t b_range;        /* multiplier of a in the key */
+} qk_batch_columns;
+
+typedef struct qk_batch_query {
+    int32_t first_block;    /* blocks [first_block, first_block + nblocks) */
+    int32_t nblocks;
+    int64_t ts_lo, ts_hi;   /* WHERE ts >= ts_lo AND ts < ts_hi */
+    int32_t key_mod;        /* groups: 1 .. 2^24 */
+    int32_t reserved;       /* 0 */
+} qk_batch_query;
+
+typedef struct qk_batch_tuning {
+    int32_t tile_rows;      /* rows per work unit (multiple of 4096) */
+    int32_t lds_groups;     /* largest key_mod aggregated in LDS */
+    int32_t wgs_per_cu;     /* persistent workgroups per CU (1 or 2) */
+    int32_t use_graph;      /* replay the batch as a captured hipGraph */
+} qk_batch_tuning;
+
+typedef struct qk_batch_s* qk_batch;
+
+/* A batch engine over one table (columns stay owned by the caller). */
+int qk_batch_create(qk_batch* out, const qk_batch_columns* cols);
+int qk_batch_destroy(qk_batch b);
+int qk_batch_set_tuning(qk_batch b, const qk_batch_tuning* t);
+int qk_batch_get_tuning(qk_batch b, qk_batch_tuning* t);
+/* Time candidate tunings on the given queries (they are run several times) and keep the
+ * fastest; *out (may be NULL) receives it. */
+int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_tuning* out,
+                      void* stream);
+/* Run nq queries.  Results stay in the engine until the next run: for query i, groups
+ * g in [0, key_mod_i) are at offset off[i] + g of the four device arrays (a group exists
+ * when its count is non-zero; MIN/MAX of an empty group are undefined).  Returns after
+ * the work is queued; qk_stream_synchronize(stream) waits for it. */
+int qk_batch_run(qk_batch b, int32_t nq, const qk_batch_query* q, void* stream);
+/* Device arrays of the last run and the host array of offsets (nq + 1 entries). */
+int qk_batch_results(qk_batch b, const int64_t** count, const int64_t** sum, const int64_t** min,
+                     const int64_t** max, const int64_t** offsets_host);
+/* Completion time of each query of the last run, in ms from the start of the run (waits
+ * for the run). */
+int qk_batch_completion_ms(qk_batch b, double* out, int32_t nq);
+
+/* key[i] = (a[i] * b_range + b[i]) mod key_mod as int64: the key column a one-query-at-a-
+ * time caller materialises before GROUP BY. */
+int qk_batch_key_column(const int32_t* a, const int32_t* b, int32_t b_range, int32_t key_mod,
+                        int32_t n, int64_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

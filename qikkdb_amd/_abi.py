@@ -67,6 +67,44 @@ class Predicate(C.Structure):
     ]
 
 
+class BatchColumns(C.Structure):
+    """qk_batch_columns: the device-resident table of a batch engine."""
+
+    _fields_ = [
+        ("a", C.c_void_p),
+        ("b", C.c_void_p),
+        ("ts", C.c_void_p),
+        ("value", C.c_void_p),
+        ("rows", C.c_int64),
+        ("block_rows", C.c_int32),
+        ("b_range", C.c_int32),
+    ]
+
+
+class BatchQuery(C.Structure):
+    """qk_batch_query: one filtered GROUP BY of a batch."""
+
+    _fields_ = [
+        ("first_block", C.c_int32),
+        ("nblocks", C.c_int32),
+        ("ts_lo", C.c_int64),
+        ("ts_hi", C.c_int64),
+        ("key_mod", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class BatchTuning(C.Structure):
+    """qk_batch_tuning."""
+
+    _fields_ = [
+        ("tile_rows", C.c_int32),
+        ("lds_groups", C.c_int32),
+        ("wgs_per_cu", C.c_int32),
+        ("use_graph", C.c_int32),
+    ]
+
+
 _vp, _i32, _i64, _u32, _sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_size_t
 _P = C.POINTER
 
@@ -113,6 +151,15 @@ PROTOTYPES = {
     "qk_gb_set_path": [_vp, _i32, _i64],
     "qk_gb_set_defer_rows": [_vp, _i64],
     "qk_gb_stats": [_vp, _P(_i64), _i32, _vp],
+    "qk_batch_create": [_P(_vp), _P(BatchColumns)],
+    "qk_batch_destroy": [_vp],
+    "qk_batch_set_tuning": [_vp, _P(BatchTuning)],
+    "qk_batch_get_tuning": [_vp, _P(BatchTuning)],
+    "qk_batch_autotune": [_vp, _i32, _P(BatchQuery), _P(BatchTuning), _vp],
+    "qk_batch_run": [_vp, _i32, _P(BatchQuery), _vp],
+    "qk_batch_results": [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp)],
+    "qk_batch_completion_ms": [_vp, _P(C.c_double), _i32],
+    "qk_batch_key_column": [_vp, _vp, _i32, _i32, _i32, _vp, _vp],
 }
 
 _lib = None

@@ -765,7 +765,9 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     const int lane = tid & 63, wave = tid >> 6, nwaves = bd >> 6;
     for (int r0 = 0; r0 < a.W; r0 += nwaves) {
         __syncthreads();
-        const bool flush = misc[2] > 0 && (uint64_t)misc[2] * 8 > misc[3];
+        // read into scalar registers: a barrier-containing branch on an LDS value must be
+        // visibly uniform, or the compiler may split the wave around it
+        const bool flush = __builtin_amdgcn_readfirstlane((int)(misc[2] > 0 && (uint64_t)misc[2] * 8 > misc[3])) != 0;
         __syncthreads();
         if (flush) merge();
         if (tid == 0) {
