@@ -310,7 +310,9 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
  *     key = (a * b_range + b) mod key_mod        (Euclidean remainder, 0 <= key < key_mod)
  * of two int32 columns, so its groups are the dense range [0, key_mod).  A query with at
  * most `lds_groups` groups aggregates each work unit in LDS and adds the unit's groups to
- * its result with atomics; a larger one aggregates with device atomics directly.
+ * its result with atomics.  A larger query takes two passes: its passing rows are first
+ * written to a scratch pool grouped by key chunk, then each chunk is aggregated in LDS by one
+ * workgroup (rows that find no room in the pool are aggregated with device atomics).
  * ====================================================================================== */
 
 typedef struct qk_batch_columns {
@@ -333,9 +335,10 @@ typedef struct qk_batch_query {
 
 typedef struct qk_batch_tuning {
     int32_t tile_rows;      /* rows per work unit (multiple of 4096) */
-    int32_t lds_groups;     /* largest key_mod aggregated in LDS */
+    int32_t lds_groups;     /* largest key_mod aggregated in LDS in one pass */
     int32_t wgs_per_cu;     /* persistent workgroups per CU (1 or 2) */
     int32_t use_graph;      /* replay the batch as a captured hipGraph */
+    int64_t scratch_rows;   /* rows of the pool of the two-pass path (0 = 2^30, 12 B per row) */
 } qk_batch_tuning;
 
 typedef struct qk_batch_s* qk_batch;

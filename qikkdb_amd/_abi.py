@@ -102,6 +102,7 @@ class BatchTuning(C.Structure):
         ("lds_groups", C.c_int32),
         ("wgs_per_cu", C.c_int32),
         ("use_graph", C.c_int32),
+        ("scratch_rows", C.c_int64),
     ]
 
 

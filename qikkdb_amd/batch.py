@@ -36,10 +36,11 @@ class Query:
 
 @dataclass
 class Tuning:
-    tile_rows: int = 1 << 18
+    tile_rows: int = 1 << 20
     lds_groups: int = 4096
     wgs_per_cu: int = 1
     use_graph: int = 1
+    scratch_rows: int = 0
 
 
 def _queries(qs: Sequence[Query]):
@@ -72,18 +73,19 @@ class BatchEngine:
             pass
 
     def set_tuning(self, t: Tuning) -> None:
-        call("qk_batch_set_tuning", self.h, C.byref(BatchTuning(t.tile_rows, t.lds_groups, t.wgs_per_cu, t.use_graph)))
+        call("qk_batch_set_tuning", self.h,
+             C.byref(BatchTuning(t.tile_rows, t.lds_groups, t.wgs_per_cu, t.use_graph, t.scratch_rows)))
 
     def tuning(self) -> Tuning:
         t = BatchTuning()
         call("qk_batch_get_tuning", self.h, C.byref(t))
-        return Tuning(t.tile_rows, t.lds_groups, t.wgs_per_cu, t.use_graph)
+        return Tuning(t.tile_rows, t.lds_groups, t.wgs_per_cu, t.use_graph, t.scratch_rows)
 
     def autotune(self, qs: Sequence[Query], stream=None) -> Tuning:
         arr, n = _queries(qs)
         t = BatchTuning()
         call("qk_batch_autotune", self.h, n, arr, C.byref(t), stream)
-        return Tuning(t.tile_rows, t.lds_groups, t.wgs_per_cu, t.use_graph)
+        return Tuning(t.tile_rows, t.lds_groups, t.wgs_per_cu, t.use_graph, t.scratch_rows)
 
     def run(self, qs: Sequence[Query], stream=None) -> None:
         arr, n = _queries(qs)

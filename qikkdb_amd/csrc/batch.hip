@@ -1,20 +1,25 @@
-// batch.hip - many independent filtered GROUP BY queries over one device-resident table in
-// one persistent kernel (qk_batch_* in include/qikkdb_amd.h).
+// batch.hip - many independent filtered GROUP BY queries over one device-resident table
+// (qk_batch_* in include/qikkdb_amd.h).
 //
 // Query i reads rows [row0_i, row0_i + rows_i) of columns a, b (int32), ts and value (int64),
 // keeps rows with lo_i <= ts < hi_i and groups them by key = (a * b_range + b) mod key_mod_i,
 // a dense range [0, key_mod_i).  Its rows are cut into work units of `tile_rows` rows; the
 // units of all queries form one range that persistent workgroups claim from a device counter.
 //
-// A unit of a query with at most `lds_groups` groups aggregates in a dense LDS table (count,
-// sum, min, max per group) and then adds every touched group to the query's result with
-// device atomics.  A unit of a larger query updates the result with device atomics per row
-// (MIN/MAX only when they improve).  Results are dense arrays indexed by out_i + key; MIN/MAX
-// are kept in an order-preserving unsigned encoding during the run and decoded at its end.
+// Small queries (key_mod <= lds_groups) aggregate a unit in a dense LDS table and add the
+// unit's groups to the result with device atomics.
 //
-// The run is init (clear results and counters) -> persistent kernel -> finalize (decode),
-// all reading the batch's size from a device header so that one captured hipGraph serves
-// every batch that fits the engine's buffers.
+// Large queries take two passes instead of per-row device atomics.  Pass 1 counts a unit's
+// passing rows per key chunk (chunk = key >> chunk_bits) in LDS, takes one slice of a scratch
+// pool for all of them, and writes the rows (key, value) into the slice grouped by chunk,
+// recording where each chunk starts.  Pass 2 gives each (query, chunk) to one workgroup, which
+// aggregates the chunk's rows from every unit of the query in a dense LDS table and adds the
+// table to the result with plain read-modify-write (it is the only writer of those groups).
+// Large queries are scheduled in waves whose estimated rows fit the pool; a unit whose rows do
+// not fit takes the atomic path.  Rows per query are estimated by sampling before the run.
+//
+// Results are dense arrays indexed by out_i + key.  MIN/MAX are kept in an order-preserving
+// unsigned encoding during the run and decoded at its end.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -25,23 +30,37 @@ namespace qk {
 namespace {
 
 constexpr int kBThreads = 1024;
-constexpr int kBRows = 4;  // rows per thread per step of a unit
+constexpr int kBRows = 4;            // rows per thread per step of a unit
+constexpr int kSamples = 4096;       // rows sampled per large query for its estimate
+constexpr int kMaxKeyBits = 24;      // key_mod <= 2^24
 
 struct BQuery {
     int64_t row0, rows;   // table rows of the query
-    int64_t unit0;        // first work unit
+    int64_t unit0;        // first pass-1 unit
     int64_t ts_lo, ts_hi;
     int64_t out;          // offset of group 0 in the result arrays
     uint64_t magic;       // ~0 / key_mod + 1 (remainder by multiplication)
+    int64_t desc;         // large: first chunk offset of its units (units x (chunks + 1) int32)
+    int64_t segb;         // large: first slice base of its units (int64 per unit)
+    int64_t chunk0;       // large: first pass-2 unit
     int32_t mod;
     int32_t units;
+    int32_t chunks;       // large: key chunks; 0 for a small query
+    int32_t index;        // position in the caller's list
+};
+
+struct BWave {
+    int64_t unit_begin, unit_end;    // pass-1 units
+    int64_t chunk_begin, chunk_end;  // pass-2 units
+    int32_t q_begin, q_end;          // queries of the wave (plan order)
 };
 
 struct BHeader {
-    int32_t nq;
-    int32_t pad;
-    int64_t units;
+    int32_t nq, nwaves;
     int64_t out_total;
+    int64_t scratch_cap;  // rows
+    int32_t chunk_bits;
+    int32_t pad;
 };
 
 struct BArgs {
@@ -53,11 +72,17 @@ struct BArgs {
     int32_t tile_rows;
     int32_t lds_groups;
     const BHeader* hdr;
+    const BWave* waves;
     const BQuery* q;
-    unsigned long long* next_unit;
-    uint32_t* done;             // [nq] finished units
-    unsigned long long* comp;   // [nq] completion clock
+    unsigned long long* ctr;    // per wave: [3w] pass-1 units, [3w+1] pass-2 units, [3w+2] pool top
     unsigned long long* t0;     // start clock
+    uint32_t* done;             // [nq] finished units (pass 1 for small, pass 2 for large queries)
+    unsigned long long* comp;   // [nq, caller order] completion clock
+    int32_t* desc;
+    int64_t* segb;
+    uint32_t* sk;               // scratch pool: keys
+    int64_t* sv;                // scratch pool: values
+    int64_t* est;               // [nq] estimated passing rows (sampling)
     unsigned long long* cnt;
     unsigned long long* sum;
     unsigned long long* mn;
@@ -72,35 +97,6 @@ __device__ __forceinline__ uint32_t key_of(int32_t a, int32_t b, int32_t b_range
     int64_t r = x % q.mod;
     if (r < 0) r += q.mod;
     return (uint32_t)r;
-}
-
-__global__ void __launch_bounds__(256) k_batch_init(BArgs p) {
-    const BHeader h = *p.hdr;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint64_t i = i0; i < (uint64_t)h.out_total; i += stride) {
-        p.cnt[i] = 0;
-        p.sum[i] = 0;
-        p.mn[i] = ~0ull;
-        p.mx[i] = 0;
-    }
-    for (uint64_t i = i0; i < (uint64_t)h.nq; i += stride) {
-        p.done[i] = 0;
-        p.comp[i] = 0;
-    }
-    if (i0 == 0) {
-        *p.next_unit = 0;
-        *p.t0 = wall_clock64();
-    }
-}
-
-__global__ void __launch_bounds__(256) k_batch_finalize(BArgs p) {
-    const BHeader h = *p.hdr;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)h.out_total; i += stride) {
-        p.mn[i] = (unsigned long long)dec_i64(p.mn[i]);
-        p.mx[i] = (unsigned long long)dec_i64(p.mx[i]);
-    }
 }
 
 // A value every lane holds the same copy of, moved to scalar registers so that branches on it
@@ -120,42 +116,136 @@ __device__ __forceinline__ void add_global(const BArgs& p, int64_t at, uint32_t 
     if (mx > *(volatile unsigned long long*)(p.mx + at)) atomicMax(p.mx + at, (unsigned long long)mx);
 }
 
-__global__ void __launch_bounds__(kBThreads) k_batch_run(BArgs p) {
+__device__ __forceinline__ void mark_done(const BArgs& p, int32_t q, uint32_t of) {
+    const uint32_t d = atomicAdd(p.done + q, 1u) + 1;
+    if (d == of) p.comp[p.q[q].index] = wall_clock64();
+}
+
+// Exclusive prefix sum over a workgroup (blockDim a multiple of 64); *total gets the sum.
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(x, d, 64);
+        if (lane >= d) x += t;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < nw; w++) {
+        if (w < wid) before += wsum[w];
+        all += wsum[w];
+    }
+    *total = all;
+    __syncthreads();
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(256) k_batch_init(BArgs p) {
+    const BHeader h = *p.hdr;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = i0; i < (uint64_t)h.out_total; i += stride) {
+        p.cnt[i] = 0;
+        p.sum[i] = 0;
+        p.mn[i] = ~0ull;
+        p.mx[i] = 0;
+    }
+    for (uint64_t i = i0; i < (uint64_t)h.nq; i += stride) {
+        p.done[i] = 0;
+        p.comp[i] = 0;
+    }
+    for (uint64_t i = i0; i < 3 * (uint64_t)h.nwaves; i += stride) p.ctr[i] = 0;
+    if (i0 == 0) *p.t0 = wall_clock64();
+}
+
+__global__ void __launch_bounds__(256) k_batch_finalize(BArgs p) {
+    const BHeader h = *p.hdr;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (uint64_t)h.out_total; i += stride) {
+        p.mn[i] = (unsigned long long)dec_i64(p.mn[i]);
+        p.mx[i] = (unsigned long long)dec_i64(p.mx[i]);
+    }
+}
+
+// One workgroup per query: passing rows among kSamples rows spread over its range, scaled.
+__global__ void __launch_bounds__(256) k_batch_estimate(BArgs p, int32_t nq) {
+    __shared__ uint32_t hits;
+    const int32_t i = blockIdx.x;
+    if (i >= nq) return;
+    const BQuery Q = p.q[i];
+    if (threadIdx.x == 0) hits = 0;
+    __syncthreads();
+    const int64_t stride = max<int64_t>(1, Q.rows / kSamples);
+    uint32_t mine = 0;
+    for (int s = threadIdx.x; s < kSamples; s += blockDim.x) {
+        const int64_t r = Q.row0 + min<int64_t>((int64_t)s * stride, Q.rows - 1);
+        const int64_t t = p.ts[r];
+        mine += t >= Q.ts_lo && t < Q.ts_hi;
+    }
+    atomicAdd(&hits, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) p.est[i] = (int64_t)(((double)hits / kSamples) * (double)Q.rows);
+}
+
+template <typename F>
+__device__ __forceinline__ void for_passing_rows(const BArgs& p, const BQuery& Q, int64_t r_begin, int64_t r_end,
+                                                 F f) {
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
+        int64_t tsv[kBRows];
+#pragma unroll
+        for (int k = 0; k < kBRows; k++) {
+            const int64_t r = min(base + (int64_t)k * bd + tid, r_end - 1);
+            tsv[k] = p.ts[r];
+        }
+#pragma unroll
+        for (int k = 0; k < kBRows; k++) {
+            const int64_t r = base + (int64_t)k * bd + tid;
+            if (r >= r_end || tsv[k] < Q.ts_lo || tsv[k] >= Q.ts_hi) continue;
+            f(r, key_of(p.a[r], p.b[r], p.b_range, Q));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave) {
     extern __shared__ __align__(16) uint64_t lds[];
-    __shared__ int64_t s_unit;
+    __shared__ int64_t s_unit, s_base;
     __shared__ int32_t s_q;
+    __shared__ uint32_t wsum[16];
     const int tid = threadIdx.x, bd = blockDim.x;
     const BHeader h = *p.hdr;
+    const BWave W = p.waves[wave];
     const int G = p.lds_groups;
-    int64_t* lsum = (int64_t*)lds;
-    uint64_t* lmn = (uint64_t*)(lsum + G);
-    uint64_t* lmx = lmn + G;
-    uint32_t* lcnt = (uint32_t*)(lmx + G);
-    int32_t q = 0;  // units are claimed in increasing order, so a workgroup's query index only grows
-    int32_t prev_q = -1, prev_units = 0;
+    const int cbits = h.chunk_bits;
+    int32_t q = W.q_begin;  // units are claimed in increasing order, so the query index only grows
+    int32_t prev_q = -1;
+    uint32_t prev_units = 0;
     while (true) {
         if (tid == 0) {
-            // the previous unit's updates were issued before the barrier that ended it
-            if (prev_q >= 0) {
-                const uint32_t d = atomicAdd(p.done + prev_q, 1u) + 1;
-                if (d == (uint32_t)prev_units) p.comp[prev_q] = wall_clock64();
-            }
-            const int64_t u = (int64_t)atomicAdd(p.next_unit, 1ull);
+            if (prev_q >= 0) mark_done(p, prev_q, prev_units);
+            const int64_t u = W.unit_begin + (int64_t)atomicAdd(p.ctr + 3 * wave, 1ull);
             s_unit = u;
-            if (u < h.units) {
-                while (q + 1 < h.nq && p.q[q + 1].unit0 <= u) q++;
+            if (u < W.unit_end) {
+                while (q + 1 < W.q_end && p.q[q + 1].unit0 <= u) q++;
                 s_q = q;
             }
         }
         __syncthreads();
         const int64_t u = uniform64(s_unit);
-        if (u >= h.units) break;  // every workgroup ends here once the counter passes the last unit
+        if (u >= W.unit_end) break;  // every workgroup ends here once the counter passes the wave
         q = (int32_t)uniform64(s_q);
         const BQuery Q = p.q[q];
         const int64_t r_begin = Q.row0 + (u - Q.unit0) * (int64_t)p.tile_rows;
         const int64_t r_end = min(r_begin + (int64_t)p.tile_rows, Q.row0 + Q.rows);
-        const bool in_lds = Q.mod <= G;
-        if (in_lds) {
+        prev_q = -1;
+        if (Q.chunks == 0) {
+            // small query: dense LDS table of key_mod groups
+            int64_t* lsum = (int64_t*)lds;
+            uint64_t* lmn = (uint64_t*)(lsum + G);
+            uint64_t* lmx = lmn + G;
+            uint32_t* lcnt = (uint32_t*)(lmx + G);
             for (int g = tid; g < Q.mod; g += bd) {
                 lsum[g] = 0;
                 lmn[g] = ~0ull;
@@ -163,45 +253,140 @@ __global__ void __launch_bounds__(kBThreads) k_batch_run(BArgs p) {
                 lcnt[g] = 0;
             }
             __syncthreads();
-        }
-        for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
-            int64_t tsv[kBRows];
-#pragma unroll
-            for (int k = 0; k < kBRows; k++) {
-                const int64_t r = min(base + (int64_t)k * bd + tid, r_end - 1);
-                tsv[k] = p.ts[r];
-            }
-#pragma unroll
-            for (int k = 0; k < kBRows; k++) {
-                const int64_t r = base + (int64_t)k * bd + tid;
-                if (r >= r_end || tsv[k] < Q.ts_lo || tsv[k] >= Q.ts_hi) continue;
-                const uint32_t key = key_of(p.a[r], p.b[r], p.b_range, Q);
+            for_passing_rows(p, Q, r_begin, r_end, [&](int64_t r, uint32_t key) {
                 const int64_t val = p.v[r];
-                if (in_lds) {
-                    atomicAdd(&lcnt[key], 1u);
-                    atomicAdd((unsigned long long*)&lsum[key], (unsigned long long)val);
-                    const uint64_t e = enc_i64(val);
-                    atomicMin((unsigned long long*)&lmn[key], (unsigned long long)e);
-                    atomicMax((unsigned long long*)&lmx[key], (unsigned long long)e);
+                const uint64_t e = enc_i64(val);
+                atomicAdd(&lcnt[key], 1u);
+                atomicAdd((unsigned long long*)&lsum[key], (unsigned long long)val);
+                atomicMin((unsigned long long*)&lmn[key], (unsigned long long)e);
+                atomicMax((unsigned long long*)&lmx[key], (unsigned long long)e);
+            });
+            __syncthreads();
+            for (int g = tid; g < Q.mod; g += bd)
+                if (lcnt[g]) add_global(p, Q.out + g, lcnt[g], lsum[g], lmn[g], lmx[g]);
+            prev_q = q;
+            prev_units = (uint32_t)Q.units;
+        } else {
+            // large query, pass 1: rows per chunk, one slice of the pool, rows written by chunk
+            const int nch = Q.chunks;
+            uint32_t* hist = (uint32_t*)lds;
+            uint32_t* cur = hist + nch;
+            for (int c = tid; c < nch; c += bd) hist[c] = 0;
+            __syncthreads();
+            for_passing_rows(p, Q, r_begin, r_end, [&](int64_t, uint32_t key) { atomicAdd(&hist[key >> cbits], 1u); });
+            __syncthreads();
+            // exclusive offsets: thread t owns chunks [t * per, (t + 1) * per)
+            const int per = (nch + bd - 1) / bd;
+            const int c0 = min(nch, tid * per), c1 = min(nch, c0 + per);
+            uint32_t run = 0;
+            for (int c = c0; c < c1; c++) run += hist[c];
+            uint32_t total;
+            uint32_t off = block_scan(run, wsum, &total);
+            int32_t* d = p.desc + Q.desc + (u - Q.unit0) * (int64_t)(nch + 1);
+            for (int c = c0; c < c1; c++) {
+                cur[c] = off;
+                d[c] = (int32_t)off;
+                off += hist[c];
+            }
+            if (tid == 0) {
+                d[nch] = (int32_t)total;
+                const int64_t base = total ? (int64_t)atomicAdd(p.ctr + 3 * wave + 2, (unsigned long long)total) : 0;
+                s_base = (base + (int64_t)total <= h.scratch_cap) ? base : -1;
+                p.segb[Q.segb + (u - Q.unit0)] = s_base;
+            }
+            __syncthreads();
+            const int64_t sbase = uniform64(s_base);
+            for_passing_rows(p, Q, r_begin, r_end, [&](int64_t r, uint32_t key) {
+                const int64_t val = p.v[r];
+                if (sbase >= 0) {
+                    const uint32_t at = atomicAdd(&cur[key >> cbits], 1u);
+                    p.sk[sbase + at] = key;
+                    p.sv[sbase + at] = val;
                 } else {
                     const uint64_t e = enc_i64(val);
                     add_global(p, Q.out + key, 1u, val, e, e);
                 }
-            }
-        }
-        if (in_lds) {
-            __syncthreads();
-            for (int g = tid; g < Q.mod; g += bd)
-                if (lcnt[g]) add_global(p, Q.out + g, lcnt[g], lsum[g], lmn[g], lmx[g]);
+            });
         }
         __syncthreads();  // the unit is done (and its LDS free) for every thread
+    }
+    if (tid == 0 && prev_q >= 0) mark_done(p, prev_q, prev_units);
+}
+
+// Pass 2: one (large query, key chunk) per unit.
+__global__ void __launch_bounds__(kBThreads) k_batch_pass2(BArgs p, int32_t wave) {
+    extern __shared__ __align__(16) uint64_t lds[];
+    __shared__ int64_t s_unit;
+    __shared__ int32_t s_q;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const BHeader h = *p.hdr;
+    const BWave W = p.waves[wave];
+    const int cbits = h.chunk_bits;
+    const int S = 1 << cbits;
+    int64_t* lsum = (int64_t*)lds;
+    uint64_t* lmn = (uint64_t*)(lsum + S);
+    uint64_t* lmx = lmn + S;
+    uint32_t* lcnt = (uint32_t*)(lmx + S);
+    int32_t q = W.q_begin;
+    int32_t prev_q = -1;
+    uint32_t prev_chunks = 0;
+    while (true) {
+        if (tid == 0) {
+            if (prev_q >= 0) mark_done(p, prev_q, prev_chunks);
+            const int64_t c = W.chunk_begin + (int64_t)atomicAdd(p.ctr + 3 * wave + 1, 1ull);
+            s_unit = c;
+            if (c < W.chunk_end) {
+                while (q < W.q_end && (p.q[q].chunks == 0 || p.q[q].chunk0 + p.q[q].chunks <= c)) q++;
+                s_q = q;
+            }
+        }
+        __syncthreads();
+        const int64_t c = uniform64(s_unit);
+        if (c >= W.chunk_end) break;
+        q = (int32_t)uniform64(s_q);
+        const BQuery Q = p.q[q];
+        const int j = (int)(c - Q.chunk0);
+        const int nch = Q.chunks;
+        const int32_t key0 = j << cbits;
+        const int32_t width = min(S, Q.mod - key0);
+        for (int g = tid; g < width; g += bd) {
+            lsum[g] = 0;
+            lmn[g] = ~0ull;
+            lmx[g] = 0;
+            lcnt[g] = 0;
+        }
+        __syncthreads();
+        // every thread walks the chunk's rows of its own units
+        for (int k = tid; k < Q.units; k += bd) {
+            const int64_t base = p.segb[Q.segb + k];
+            if (base < 0) continue;  // that unit's rows took the atomic path
+            const int32_t* d = p.desc + Q.desc + (int64_t)k * (nch + 1);
+            const int32_t lo = d[j], hi = d[j + 1];
+            for (int32_t i = lo; i < hi; i++) {
+                const uint32_t g = p.sk[base + i] - (uint32_t)key0;
+                const int64_t val = p.sv[base + i];
+                const uint64_t e = enc_i64(val);
+                atomicAdd(&lcnt[g], 1u);
+                atomicAdd((unsigned long long*)&lsum[g], (unsigned long long)val);
+                atomicMin((unsigned long long*)&lmn[g], (unsigned long long)e);
+                atomicMax((unsigned long long*)&lmx[g], (unsigned long long)e);
+            }
+        }
+        __syncthreads();
+        // this workgroup is the only writer of these groups in this kernel
+        for (int g = tid; g < width; g += bd) {
+            if (!lcnt[g]) continue;
+            const int64_t at = Q.out + key0 + g;
+            p.cnt[at] += lcnt[g];
+            p.sum[at] += (unsigned long long)lsum[g];
+            if (lmn[g] < p.mn[at]) p.mn[at] = lmn[g];
+            if (lmx[g] > p.mx[at]) p.mx[at] = lmx[g];
+        }
+        __syncthreads();
         prev_q = q;
-        prev_units = Q.units;
+        prev_chunks = (uint32_t)nch;
     }
-    if (tid == 0 && prev_q >= 0) {
-        const uint32_t d = atomicAdd(p.done + prev_q, 1u) + 1;
-        if (d == (uint32_t)prev_units) p.comp[prev_q] = wall_clock64();
-    }
+    if (tid == 0 && prev_q >= 0) mark_done(p, prev_q, prev_chunks);
 }
 
 __global__ void __launch_bounds__(256) k_key_column(const int32_t* __restrict__ a, const int32_t* __restrict__ b,
@@ -213,6 +398,22 @@ __global__ void __launch_bounds__(256) k_key_column(const int32_t* __restrict__ 
 
 uint64_t magic_of(int32_t mod) { return ~0ull / (uint64_t)(uint32_t)mod + 1; }
 
+// Key chunk of the two-pass path: the largest power of two <= max(lds_groups, 1024).
+int chunk_bits_of(int32_t lds_groups) {
+    int b = 10;
+    while (b < kMaxKeyBits && (1 << (b + 1)) <= lds_groups) b++;
+    return b;
+}
+
+size_t pass1_lds_bytes(const qk_batch_tuning& t) {
+    const int cb = chunk_bits_of(t.lds_groups);
+    const size_t small = (size_t)t.lds_groups * 28;
+    const size_t large = (size_t)8 << (kMaxKeyBits - cb);  // hist + cursors for 2^24 keys
+    return std::max(small, large) + 64;
+}
+
+size_t pass2_lds_bytes(const qk_batch_tuning& t) { return ((size_t)28 << chunk_bits_of(t.lds_groups)) + 64; }
+
 }  // namespace
 }  // namespace qk
 
@@ -221,23 +422,34 @@ using namespace qk;
 struct qk_batch_s {
     qk_batch_columns cols{};
     qk_batch_tuning tune{};
-    int device = 0;
+    int64_t scratch_rows = (int64_t)1 << 30;
     // device buffers (grown as needed)
-    void* plan = nullptr;  // BHeader + BQuery[nq]
-    size_t plan_bytes = 0;
+    void* plan = nullptr;  // BHeader + BWave[] + BQuery[]
+    size_t plan_cap = 0;
     uint32_t* done = nullptr;
     unsigned long long* comp = nullptr;
+    int64_t* est = nullptr;
     int32_t q_cap = 0;
-    unsigned long long* counters = nullptr;  // [0] next unit, [1] start clock
+    unsigned long long* ctr = nullptr;  // [0] start clock, then 3 per wave
+    int32_t wave_cap = 0;
+    int32_t* desc = nullptr;
+    int64_t desc_cap = 0;
+    int64_t* segb = nullptr;
+    int64_t segb_cap = 0;
+    uint32_t* sk = nullptr;
+    int64_t* sv = nullptr;
+    int64_t sk_cap = 0;
     unsigned long long *cnt = nullptr, *sum = nullptr, *mn = nullptr, *mx = nullptr;
     int64_t out_cap = 0;
     void* host_plan = nullptr;  // pinned copy of the plan
-    size_t host_plan_bytes = 0;
+    size_t host_plan_cap = 0;
     std::vector<int64_t> offsets;
+    std::vector<int64_t> est_host;
     int32_t last_nq = 0;
-    // captured graph of init -> run -> finalize for the current buffers and tuning
+    // captured graph of init -> waves -> finalize, valid while the launch arguments match
     hipGraphExec_t graph = nullptr;
-    bool graph_ok = false;
+    int32_t graph_waves = -1;
+    BArgs graph_args{};
 };
 
 namespace qk {
@@ -246,63 +458,46 @@ namespace {
 void free_graph(qk_batch_s* b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
-    b->graph_ok = false;
+    b->graph_waves = -1;
+}
+
+template <typename T>
+int grow(T** p, int64_t* cap, int64_t need, int64_t min_cap) {
+    if (*p && *cap >= need) return QK_SUCCESS;
+    if (*p) pool_free(*p);
+    *p = nullptr;
+    *cap = 0;
+    const int64_t c = std::max(need, min_cap);
+    QK_TRY(pool_alloc((void**)p, (size_t)c * sizeof(T)));
+    *cap = c;
+    return QK_SUCCESS;
 }
 
 void free_buffers(qk_batch_s* b) {
-    void* ptrs[] = {b->plan, b->done, b->comp, b->counters, b->cnt, b->sum, b->mn, b->mx};
+    void* ptrs[] = {b->plan, b->done, b->comp, b->est, b->ctr, b->desc, b->segb, b->sk, b->sv,
+                    b->cnt, b->sum, b->mn, b->mx};
     for (void* p : ptrs)
         if (p) pool_free(p);
     b->plan = nullptr;
     b->done = nullptr;
     b->comp = nullptr;
-    b->counters = nullptr;
+    b->est = nullptr;
+    b->ctr = nullptr;
+    b->desc = nullptr;
+    b->segb = nullptr;
+    b->sk = nullptr;
+    b->sv = nullptr;
     b->cnt = b->sum = b->mn = b->mx = nullptr;
-    b->plan_bytes = 0;
-    b->q_cap = 0;
-    b->out_cap = 0;
+    b->plan_cap = 0;
+    b->q_cap = b->wave_cap = 0;
+    b->desc_cap = b->segb_cap = b->sk_cap = b->out_cap = 0;
     if (b->host_plan) (void)hipHostFree(b->host_plan);
     b->host_plan = nullptr;
-    b->host_plan_bytes = 0;
+    b->host_plan_cap = 0;
     free_graph(b);
 }
 
-int ensure_buffers(qk_batch_s* b, int32_t nq, int64_t out_total) {
-    const size_t pbytes = sizeof(BHeader) + sizeof(BQuery) * (size_t)std::max(nq, 1);
-    if (nq <= b->q_cap && out_total <= b->out_cap && pbytes <= b->plan_bytes && b->counters) return QK_SUCCESS;
-    const int32_t qc = std::max(nq, std::max(b->q_cap * 2, 1024));
-    const int64_t oc = std::max(out_total, std::max(b->out_cap + b->out_cap / 2, (int64_t)1 << 20));
-    free_buffers(b);
-    int st = QK_SUCCESS;
-    auto A = [&](void** p, size_t bytes) {
-        if (st == QK_SUCCESS) st = pool_alloc(p, bytes);
-    };
-    const size_t pb = sizeof(BHeader) + sizeof(BQuery) * (size_t)qc;
-    A(&b->plan, pb);
-    A((void**)&b->done, (size_t)qc * 4);
-    A((void**)&b->comp, (size_t)qc * 8);
-    A((void**)&b->counters, 64);
-    A((void**)&b->cnt, (size_t)oc * 8);
-    A((void**)&b->sum, (size_t)oc * 8);
-    A((void**)&b->mn, (size_t)oc * 8);
-    A((void**)&b->mx, (size_t)oc * 8);
-    if (st != QK_SUCCESS) {
-        free_buffers(b);
-        return st;
-    }
-    if (hipHostMalloc(&b->host_plan, pb) != hipSuccess) {
-        b->host_plan = nullptr;
-        free_buffers(b);
-        return fail(QK_OUT_OF_MEMORY, "qk_batch: pinned plan buffer");
-    }
-    b->host_plan_bytes = pb;
-    b->plan_bytes = pb;
-    b->q_cap = qc;
-    b->out_cap = oc;
-    return QK_SUCCESS;
-}
-
-BArgs args_of(const qk_batch_s* b) {
+BArgs args_of(const qk_batch_s* b, int32_t nwaves) {
     BArgs p;
     p.a = b->cols.a;
     p.b = b->cols.b;
@@ -312,11 +507,17 @@ BArgs args_of(const qk_batch_s* b) {
     p.tile_rows = b->tune.tile_rows;
     p.lds_groups = b->tune.lds_groups;
     p.hdr = (const BHeader*)b->plan;
-    p.q = (const BQuery*)((const char*)b->plan + sizeof(BHeader));
-    p.next_unit = b->counters;
-    p.t0 = b->counters + 1;
+    p.waves = (const BWave*)((const char*)b->plan + sizeof(BHeader));
+    p.q = (const BQuery*)((const char*)b->plan + sizeof(BHeader) + sizeof(BWave) * (size_t)nwaves);
+    p.t0 = b->ctr;
+    p.ctr = b->ctr + 1;
     p.done = b->done;
     p.comp = b->comp;
+    p.desc = b->desc;
+    p.segb = b->segb;
+    p.sk = b->sk;
+    p.sv = b->sv;
+    p.est = b->est;
     p.cnt = b->cnt;
     p.sum = b->sum;
     p.mn = b->mn;
@@ -324,100 +525,247 @@ BArgs args_of(const qk_batch_s* b) {
     return p;
 }
 
-size_t run_lds_bytes(const qk_batch_tuning& t) { return (size_t)t.lds_groups * 28 + 64; }
-
-int launch_all(qk_batch_s* b, hipStream_t s) {
-    const BArgs p = args_of(b);
+int launch_all(qk_batch_s* b, int32_t nwaves, hipStream_t s) {
+    const BArgs p = args_of(b, nwaves);
     const int cus = device_info().cus;
     k_batch_init<<<cus * 8, 256, 0, s>>>(p);
     QK_HIP(hipGetLastError());
-    k_batch_run<<<cus * b->tune.wgs_per_cu, kBThreads, run_lds_bytes(b->tune), s>>>(p);
-    QK_HIP(hipGetLastError());
+    for (int32_t w = 0; w < nwaves; w++) {
+        k_batch_pass1<<<cus * b->tune.wgs_per_cu, kBThreads, pass1_lds_bytes(b->tune), s>>>(p, w);
+        QK_HIP(hipGetLastError());
+        k_batch_pass2<<<cus * b->tune.wgs_per_cu, kBThreads, pass2_lds_bytes(b->tune), s>>>(p, w);
+        QK_HIP(hipGetLastError());
+    }
     k_batch_finalize<<<cus * 8, 256, 0, s>>>(p);
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }
 
-// Capture init -> run -> finalize once for the current buffers and tuning.
-int build_graph(qk_batch_s* b) {
+// Capture init -> waves -> finalize once for the current buffers, tuning and wave count.
+void build_graph(qk_batch_s* b, int32_t nwaves) {
     free_graph(b);
     hipStream_t cs = nullptr;
-    QK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    hipGraph_t g = nullptr;
-    int st = QK_SUCCESS;
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) st = QK_RUNTIME_ERROR;
-    if (st == QK_SUCCESS) {
-        const int lst = launch_all(b, cs);
-        const hipError_t e = hipStreamEndCapture(cs, &g);
-        if (lst != QK_SUCCESS || e != hipSuccess) st = QK_RUNTIME_ERROR;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
     }
-    if (st == QK_SUCCESS && hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+    hipGraph_t g = nullptr;
+    bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        const int lst = launch_all(b, nwaves, cs);
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        ok = lst == QK_SUCCESS && e == hipSuccess;
+    }
+    if (ok && hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0) != hipSuccess) {
         b->graph = nullptr;
-        st = QK_RUNTIME_ERROR;
+        ok = false;
     }
     if (g) (void)hipGraphDestroy(g);
     (void)hipStreamDestroy(cs);
     (void)hipGetLastError();
-    b->graph_ok = st == QK_SUCCESS;
-    return QK_SUCCESS;  // without a graph the kernels are launched one by one
+    b->graph_waves = ok ? nwaves : -1;  // without a graph the kernels are launched one by one
 }
 
 int check_tuning(const qk_batch_tuning& t) {
     if (t.tile_rows < 4096 || t.tile_rows % 4096) return fail(QK_ERROR, "qk_batch: tile_rows must be a positive multiple of 4096");
-    if (t.lds_groups < 0 || run_lds_bytes(t) > std::min<size_t>(device_info().lds_per_block, 160 * 1024) - 256)
+    const size_t limit = std::min<size_t>(device_info().lds_per_block, 160 * 1024) - 512;
+    if (t.lds_groups < 0 || pass1_lds_bytes(t) > limit || pass2_lds_bytes(t) > limit)
         return fail(QK_ERROR, "qk_batch: lds_groups does not fit the LDS");
     if (t.wgs_per_cu < 1 || t.wgs_per_cu > 2) return fail(QK_ERROR, "qk_batch: wgs_per_cu must be 1 or 2");
-    if (t.wgs_per_cu == 2 && run_lds_bytes(t) > std::min<size_t>(device_info().lds_per_cu, 160 * 1024) / 2 - 256)
+    const size_t half = std::min<size_t>(device_info().lds_per_cu, 160 * 1024) / 2 - 512;
+    if (t.wgs_per_cu == 2 && (pass1_lds_bytes(t) > half || pass2_lds_bytes(t) > half))
         return fail(QK_ERROR, "qk_batch: two workgroups per CU need lds_groups below half the LDS");
+    return QK_SUCCESS;
+}
+
+int ensure_plan(qk_batch_s* b, size_t bytes, int32_t nq, int32_t nwaves) {
+    if (bytes > b->plan_cap || nq > b->q_cap || nwaves > b->wave_cap) {
+        free_graph(b);
+        void* ptrs[] = {b->plan, b->done, b->comp, b->est, b->ctr};
+        for (void* p : ptrs)
+            if (p) pool_free(p);
+        b->plan = nullptr;
+        b->done = nullptr;
+        b->comp = nullptr;
+        b->est = nullptr;
+        b->ctr = nullptr;
+        if (b->host_plan) (void)hipHostFree(b->host_plan);
+        b->host_plan = nullptr;
+        const size_t pc = std::max(bytes, b->plan_cap * 2);
+        const int32_t qc = std::max(nq, std::max(b->q_cap * 2, 1024));
+        const int32_t wc = std::max(nwaves, std::max(b->wave_cap * 2, 64));
+        b->plan_cap = b->host_plan_cap = 0;
+        b->q_cap = b->wave_cap = 0;
+        QK_TRY(pool_alloc(&b->plan, pc));
+        QK_TRY(pool_alloc((void**)&b->done, (size_t)qc * 4));
+        QK_TRY(pool_alloc((void**)&b->comp, (size_t)qc * 8));
+        QK_TRY(pool_alloc((void**)&b->est, (size_t)qc * 8));
+        QK_TRY(pool_alloc((void**)&b->ctr, (size_t)(1 + 3 * wc) * 8));
+        if (hipHostMalloc(&b->host_plan, pc) != hipSuccess) {
+            b->host_plan = nullptr;
+            return fail(QK_OUT_OF_MEMORY, "qk_batch: pinned plan buffer");
+        }
+        b->plan_cap = b->host_plan_cap = pc;
+        b->q_cap = qc;
+        b->wave_cap = wc;
+    }
     return QK_SUCCESS;
 }
 
 int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s) {
     const qk_batch_columns& c = b->cols;
+    const qk_batch_tuning& t = b->tune;
     const int64_t nblocks = (c.rows + c.block_rows - 1) / c.block_rows;
-    int64_t units = 0, out = 0;
-    b->offsets.assign((size_t)nq + 1, 0);
+    const int cbits = chunk_bits_of(t.lds_groups);
+    if (nq > (1 << 24)) return fail(QK_ERROR, "qk_batch: at most 2^24 queries per run");
+    // queries in plan order: small ones first, then large ones (waves are cut below)
+    std::vector<int32_t> order;
+    order.reserve((size_t)nq);
+    for (int pass = 0; pass < 2; pass++)
+        for (int32_t i = 0; i < nq; i++)
+            if ((q[i].key_mod > t.lds_groups) == (pass == 1)) order.push_back(i);
     std::vector<BQuery> bq((size_t)nq);
+    b->offsets.assign((size_t)nq + 1, 0);
+    int64_t out = 0;
     for (int32_t i = 0; i < nq; i++) {
         const qk_batch_query& x = q[i];
-        if (x.key_mod < 1 || x.key_mod > (1 << 24)) return fail(QK_ERROR, "qk_batch: query %d: key_mod must be 1..2^24", i);
+        if (x.key_mod < 1 || x.key_mod > (1 << kMaxKeyBits))
+            return fail(QK_ERROR, "qk_batch: query %d: key_mod must be 1..2^24", i);
         if (x.first_block < 0 || x.nblocks < 1 || x.first_block + (int64_t)x.nblocks > nblocks)
             return fail(QK_ERROR, "qk_batch: query %d: block range outside the table", i);
-        BQuery& d = bq[(size_t)i];
-        d.row0 = (int64_t)x.first_block * c.block_rows;
-        d.rows = std::min<int64_t>((int64_t)(x.first_block + x.nblocks) * c.block_rows, c.rows) - d.row0;
-        d.unit0 = units;
-        d.units = (int32_t)((d.rows + b->tune.tile_rows - 1) / b->tune.tile_rows);
-        d.ts_lo = x.ts_lo;
-        d.ts_hi = x.ts_hi;
-        d.out = out;
-        d.mod = x.key_mod;
-        d.magic = magic_of(x.key_mod);
-        units += d.units;
         b->offsets[(size_t)i] = out;
         out += x.key_mod;
     }
     b->offsets[(size_t)nq] = out;
-    const bool grew = nq > b->q_cap || out > b->out_cap || !b->counters;
-    QK_TRY(ensure_buffers(b, nq, out));
-    // the plan travels through pinned memory: wait until the previous run has read it
-    QK_HIP(hipStreamSynchronize(s));
+    int64_t units = 0, desc = 0, segb = 0, chunk_units = 0;
+    int32_t nlarge = 0;
+    for (size_t k = 0; k < order.size(); k++) {
+        const int32_t i = order[k];
+        const qk_batch_query& x = q[i];
+        BQuery& d = bq[k];
+        d = BQuery{};
+        d.row0 = (int64_t)x.first_block * c.block_rows;
+        d.rows = std::min<int64_t>((int64_t)(x.first_block + x.nblocks) * c.block_rows, c.rows) - d.row0;
+        d.unit0 = units;
+        d.units = (int32_t)((d.rows + t.tile_rows - 1) / t.tile_rows);
+        d.ts_lo = x.ts_lo;
+        d.ts_hi = x.ts_hi;
+        d.out = b->offsets[(size_t)i];
+        d.mod = x.key_mod;
+        d.magic = magic_of(x.key_mod);
+        d.index = i;
+        if (x.key_mod > t.lds_groups) {
+            d.chunks = (int32_t)(((int64_t)x.key_mod + (1 << cbits) - 1) >> cbits);
+            d.desc = desc;
+            d.segb = segb;
+            d.chunk0 = chunk_units;
+            desc += (int64_t)d.units * (d.chunks + 1);
+            segb += d.units;
+            chunk_units += d.chunks;
+            nlarge++;
+        }
+        units += d.units;
+    }
+    // grow the buffers that depend on the batch
+    if (out > b->out_cap) {
+        unsigned long long** arrs[] = {&b->cnt, &b->sum, &b->mn, &b->mx};
+        const int64_t cap = std::max(out, std::max(b->out_cap + b->out_cap / 2, (int64_t)1 << 20));
+        for (unsigned long long** a : arrs) {
+            if (*a) pool_free(*a);
+            *a = nullptr;
+        }
+        b->out_cap = 0;
+        for (unsigned long long** a : arrs) QK_TRY(pool_alloc((void**)a, (size_t)cap * 8));
+        b->out_cap = cap;
+    }
+    QK_TRY(grow(&b->desc, &b->desc_cap, std::max<int64_t>(desc, 1), (int64_t)1 << 16));
+    QK_TRY(grow(&b->segb, &b->segb_cap, std::max<int64_t>(segb, 1), (int64_t)1 << 12));
+    const int64_t pool = b->scratch_rows;
+    if (nlarge && (!b->sk || !b->sv || b->sk_cap < pool)) {
+        if (b->sk) pool_free(b->sk);
+        if (b->sv) pool_free(b->sv);
+        b->sk = nullptr;
+        b->sv = nullptr;
+        b->sk_cap = 0;
+        QK_TRY(pool_alloc((void**)&b->sk, (size_t)pool * 4));
+        QK_TRY(pool_alloc((void**)&b->sv, (size_t)pool * 8));
+        b->sk_cap = pool;
+    }
+    const size_t max_plan = sizeof(BHeader) + sizeof(BWave) * (size_t)std::max(1, nlarge) + sizeof(BQuery) * (size_t)std::max(nq, 1);
+    QK_TRY(ensure_plan(b, max_plan, nq, std::max(1, nlarge)));
+    // estimate the passing rows of the large queries (sampling) to cut the waves
+    const int32_t nsmall = nq - nlarge;
+    std::vector<BWave> waves;
+    BWave w{};
+    w.q_begin = 0;
+    w.unit_begin = 0;
+    w.chunk_begin = 0;
+    QK_HIP(hipStreamSynchronize(s));  // the pinned plan is rewritten below
+    if (nlarge) {
+        BHeader hd{};
+        hd.nq = nq;
+        memcpy(b->host_plan, &hd, sizeof hd);
+        // estimate layout: header, one wave slot, queries
+        memcpy((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave), bq.data(), sizeof(BQuery) * (size_t)nq);
+        QK_HIP(hipMemcpyAsync(b->plan, b->host_plan, sizeof(BHeader) + sizeof(BWave) + sizeof(BQuery) * (size_t)nq,
+                              hipMemcpyHostToDevice, s));
+        BArgs ep = args_of(b, 1);
+        ep.q += nsmall;  // the large queries
+        k_batch_estimate<<<nlarge, 256, 0, s>>>(ep, nlarge);
+        QK_HIP(hipGetLastError());
+        b->est_host.assign((size_t)nlarge, 0);
+        QK_HIP(hipMemcpyAsync(b->est_host.data(), b->est, (size_t)nlarge * 8, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+    }
+    // waves: wave 0 holds every small query; large queries are added while their estimated
+    // rows fit the pool (a query above the pool gets a wave of its own)
+    int64_t used = 0;
+    w.q_end = nsmall;
+    w.unit_end = nsmall ? bq[(size_t)nsmall - 1].unit0 + bq[(size_t)nsmall - 1].units : 0;
+    w.chunk_end = 0;
+    for (int32_t k = nsmall; k < nq; k++) {
+        const BQuery& d = bq[(size_t)k];
+        const int64_t need = b->est_host[(size_t)(k - nsmall)] + b->est_host[(size_t)(k - nsmall)] / 4 + 4096;
+        if (used > 0 && used + need > pool) {
+            waves.push_back(w);
+            w = BWave{};
+            w.q_begin = k;
+            w.unit_begin = d.unit0;
+            w.chunk_begin = d.chunk0;
+            used = 0;
+        }
+        used += need;
+        w.q_end = k + 1;
+        w.unit_end = d.unit0 + d.units;
+        w.chunk_end = d.chunk0 + d.chunks;
+    }
+    waves.push_back(w);
+    const int32_t nw = (int32_t)waves.size();
     BHeader hd{};
     hd.nq = nq;
-    hd.units = units;
+    hd.nwaves = nw;
     hd.out_total = out;
+    hd.scratch_cap = nlarge ? pool : 0;
+    hd.chunk_bits = cbits;
+    const size_t plan_bytes = sizeof hd + sizeof(BWave) * (size_t)nw + sizeof(BQuery) * (size_t)nq;
+    QK_TRY(ensure_plan(b, plan_bytes, nq, nw));
     memcpy(b->host_plan, &hd, sizeof hd);
-    if (nq) memcpy((char*)b->host_plan + sizeof hd, bq.data(), sizeof(BQuery) * (size_t)nq);
-    QK_HIP(hipMemcpyAsync(b->plan, b->host_plan, sizeof hd + sizeof(BQuery) * (size_t)nq, hipMemcpyHostToDevice, s));
+    memcpy((char*)b->host_plan + sizeof hd, waves.data(), sizeof(BWave) * (size_t)nw);
+    if (nq) memcpy((char*)b->host_plan + sizeof hd + sizeof(BWave) * (size_t)nw, bq.data(), sizeof(BQuery) * (size_t)nq);
+    QK_HIP(hipMemcpyAsync(b->plan, b->host_plan, plan_bytes, hipMemcpyHostToDevice, s));
     b->last_nq = nq;
-    if (b->tune.use_graph) {
-        if (grew || !b->graph_ok) QK_TRY(build_graph(b));
-        if (b->graph_ok) {
+    if (t.use_graph) {
+        const BArgs now = args_of(b, nw);
+        if (b->graph_waves != nw || memcmp(&now, &b->graph_args, sizeof now) != 0) {
+            build_graph(b, nw);
+            b->graph_args = now;
+        }
+        if (b->graph_waves == nw) {
             QK_HIP(hipGraphLaunch(b->graph, s));
             return QK_SUCCESS;
         }
     }
-    return launch_all(b, s);
+    return launch_all(b, nw, s);
 }
 
 }  // namespace
@@ -435,10 +783,11 @@ int qk_batch_create(qk_batch* out, const qk_batch_columns* cols) {
     qk_batch_s* b = new (std::nothrow) qk_batch_s();
     if (!b) return fail(QK_OUT_OF_MEMORY, "qk_batch_create: host allocation");
     b->cols = *cols;
-    b->tune.tile_rows = 1 << 18;
+    b->tune.tile_rows = 1 << 20;
     b->tune.lds_groups = 4096;
     b->tune.wgs_per_cu = 1;
     b->tune.use_graph = 1;
+    b->tune.scratch_rows = 0;
     *out = b;
     return QK_SUCCESS;
 }
@@ -454,8 +803,19 @@ int qk_batch_destroy(qk_batch b) {
 int qk_batch_set_tuning(qk_batch b, const qk_batch_tuning* t) {
     if (!b || !t) return fail(QK_ERROR, "qk_batch_set_tuning: null argument");
     QK_TRY(check_tuning(*t));
+    if (t->scratch_rows < 0) return fail(QK_ERROR, "qk_batch_set_tuning: scratch_rows must be >= 0");
     const bool same = memcmp(&b->tune, t, sizeof *t) == 0;
     b->tune = *t;
+    const int64_t rows = t->scratch_rows > 0 ? t->scratch_rows : (int64_t)1 << 30;
+    if (rows != b->scratch_rows) {
+        if (b->sk) pool_free(b->sk);
+        if (b->sv) pool_free(b->sv);
+        b->sk = nullptr;
+        b->sv = nullptr;
+        b->sk_cap = 0;
+        b->scratch_rows = rows;
+        free_graph(b);
+    }
     if (!same) free_graph(b);
     return QK_SUCCESS;
 }
@@ -478,10 +838,10 @@ int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_
     hipStream_t s = as_stream(stream);
     const qk_batch_tuning keep = b->tune;
     std::vector<qk_batch_tuning> cands;
-    for (int tile : {1 << 16, 1 << 18, 1 << 20})
+    for (int tile : {1 << 18, 1 << 20, 1 << 22})
         for (int groups : {2048, 4096, 5120})
             for (int wpc : {1, 2}) {
-                qk_batch_tuning t{tile, groups, wpc, keep.use_graph};
+                qk_batch_tuning t{tile, groups, wpc, keep.use_graph, keep.scratch_rows};
                 if (check_tuning(t) == QK_SUCCESS) cands.push_back(t);
             }
     (void)qk_last_error_message();
@@ -533,7 +893,7 @@ int qk_batch_completion_ms(qk_batch b, double* out, int32_t nq) {
     std::vector<unsigned long long> comp((size_t)nq);
     unsigned long long t0 = 0;
     if (nq) QK_HIP(hipMemcpy(comp.data(), b->comp, (size_t)nq * 8, hipMemcpyDeviceToHost));
-    QK_HIP(hipMemcpy(&t0, b->counters + 1, 8, hipMemcpyDeviceToHost));
+    QK_HIP(hipMemcpy(&t0, b->ctr, 8, hipMemcpyDeviceToHost));
     int rate_khz = 0;
     QK_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0));
     const double per_ms = rate_khz > 0 ? (double)rate_khz : 100000.0;
