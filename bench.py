@@ -183,8 +183,14 @@ def run_batched(args) -> int:
     qs = batched_queries(args.queries, nblocks, seed=7)
     eng = B.BatchEngine(cols["a"].ptr, cols["b"].ptr, cols["ts"].ptr, cols["value"].ptr, n, blk, 4096)
     t0 = time.perf_counter()
-    tuned = eng.autotune(qs[:256])
-    log(f"autotuned on 256 queries in {time.perf_counter() - t0:.1f} s: {tuned}")
+    if args.batch_tuning:
+        tile, groups, wpc = (int(x) for x in args.batch_tuning.split(","))
+        eng.set_tuning(B.Tuning(tile, groups, wpc, 1))
+        tuned = eng.tuning()
+        log(f"tuning set: {tuned}")
+    else:
+        tuned = eng.autotune(qs[:256])
+        log(f"autotuned on 256 queries in {time.perf_counter() - t0:.1f} s: {tuned}")
     for _ in range(args.warmup):
         eng.run(qs)
     qk._abi.call("qk_device_synchronize")
@@ -294,6 +300,8 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=4096, help="batched workload: queries per batch")
     ap.add_argument("--plain-queries", type=int, default=128,
                     help="batched workload: queries timed on the one-at-a-time path")
+    ap.add_argument("--batch-tuning", default="",
+                    help="batched workload: 'tile_rows,lds_groups,wgs_per_cu' instead of autotuning")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's size)")
     ap.add_argument("--block-rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-rows", type=int, default=1 << 25)

@@ -3,20 +3,25 @@
 //
 // Query i reads rows [row0_i, row0_i + rows_i) of columns a, b (int32), ts and value (int64),
 // keeps rows with lo_i <= ts < hi_i and groups them by key = (a * b_range + b) mod key_mod_i,
-// a dense range [0, key_mod_i).  Its rows are cut into work units of `tile_rows` rows; the
-// units of all queries form one range that persistent workgroups claim from a device counter.
+// a dense range [0, key_mod_i).  The table is cut into tiles of `tile_rows` rows; a work unit
+// is one (tile, query) pair.  Units are listed tile by tile, so the workgroups running at one
+// time read the same rows for different queries and most of those reads hit the L2 and the
+// memory-side cache instead of HBM.  Persistent workgroups claim units from a device counter.
 //
 // Small queries (key_mod <= lds_groups) aggregate a unit in a dense LDS table and add the
 // unit's groups to the result with device atomics.
 //
 // Large queries take two passes instead of per-row device atomics.  Pass 1 counts a unit's
-// passing rows per key chunk (chunk = key >> chunk_bits) in LDS, takes one slice of a scratch
-// pool for all of them, and writes the rows (key, value) into the slice grouped by chunk,
-// recording where each chunk starts.  Pass 2 gives each (query, chunk) to one workgroup, which
-// aggregates the chunk's rows from every unit of the query in a dense LDS table and adds the
-// table to the result with plain read-modify-write (it is the only writer of those groups).
-// Large queries are scheduled in waves whose estimated rows fit the pool; a unit whose rows do
-// not fit takes the atomic path.  Rows per query are estimated by sampling before the run.
+// passing rows per key chunk (chunk = key >> chunk_bits of the query) in LDS, takes one slice
+// of a scratch pool for all of them, and writes the rows (key, value) into the slice grouped
+// by chunk, recording where each chunk starts.  Pass 2 gives each (query, chunk) to one
+// workgroup, whose wavefronts each take one unit's segment of the chunk at a time (coalesced
+// reads), aggregate it in a dense LDS table, and add the table to the result with plain
+// read-modify-write (the workgroup is the only writer of those groups).  The chunk width of a
+// query is chosen from its estimated rows so that no chunk holds much more than a fair share
+// of a pass, and large queries are scheduled in waves whose estimated rows fit the pool; a
+// unit whose rows do not fit takes the atomic path.  Rows per query are estimated by sampling
+// before the run.
 //
 // Results are dense arrays indexed by out_i + key.  MIN/MAX are kept in an order-preserving
 // unsigned encoding during the run and decoded at its end.
@@ -33,10 +38,11 @@ constexpr int kBThreads = 1024;
 constexpr int kBRows = 4;            // rows per thread per step of a unit
 constexpr int kSamples = 4096;       // rows sampled per large query for its estimate
 constexpr int kMaxKeyBits = 24;      // key_mod <= 2^24
+constexpr int kMinChunkBits = 6;     // narrowest key chunk of a large query: 64 groups
 
 struct BQuery {
     int64_t row0, rows;   // table rows of the query
-    int64_t unit0;        // first pass-1 unit
+    int64_t tile0;        // tile holding row0
     int64_t ts_lo, ts_hi;
     int64_t out;          // offset of group 0 in the result arrays
     uint64_t magic;       // ~0 / key_mod + 1 (remainder by multiplication)
@@ -44,9 +50,16 @@ struct BQuery {
     int64_t segb;         // large: first slice base of its units (int64 per unit)
     int64_t chunk0;       // large: first pass-2 unit
     int32_t mod;
-    int32_t units;
+    int32_t units;        // tiles the query touches
     int32_t chunks;       // large: key chunks; 0 for a small query
+    int32_t cbits;        // large: chunk = key >> cbits
     int32_t index;        // position in the caller's list
+    int32_t pad;
+};
+
+struct BUnit {
+    int32_t q;     // query (plan order)
+    int32_t tile;  // table tile
 };
 
 struct BWave {
@@ -59,8 +72,7 @@ struct BHeader {
     int32_t nq, nwaves;
     int64_t out_total;
     int64_t scratch_cap;  // rows
-    int32_t chunk_bits;
-    int32_t pad;
+    int64_t nunits;
 };
 
 struct BArgs {
@@ -71,9 +83,11 @@ struct BArgs {
     int32_t b_range;
     int32_t tile_rows;
     int32_t lds_groups;
+    int32_t chunk_max;          // widest key chunk of a large query (a power of two)
     const BHeader* hdr;
     const BWave* waves;
     const BQuery* q;
+    const BUnit* units;
     unsigned long long* ctr;    // per wave: [3w] pass-1 units, [3w+1] pass-2 units, [3w+2] pool top
     unsigned long long* t0;     // start clock
     uint32_t* done;             // [nq] finished units (pass 1 for small, pass 2 for large queries)
@@ -212,34 +226,30 @@ __device__ __forceinline__ void for_passing_rows(const BArgs& p, const BQuery& Q
 __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave) {
     extern __shared__ __align__(16) uint64_t lds[];
     __shared__ int64_t s_unit, s_base;
-    __shared__ int32_t s_q;
     __shared__ uint32_t wsum[16];
     const int tid = threadIdx.x, bd = blockDim.x;
     const BHeader h = *p.hdr;
     const BWave W = p.waves[wave];
     const int G = p.lds_groups;
-    const int cbits = h.chunk_bits;
-    int32_t q = W.q_begin;  // units are claimed in increasing order, so the query index only grows
     int32_t prev_q = -1;
     uint32_t prev_units = 0;
     while (true) {
         if (tid == 0) {
             if (prev_q >= 0) mark_done(p, prev_q, prev_units);
-            const int64_t u = W.unit_begin + (int64_t)atomicAdd(p.ctr + 3 * wave, 1ull);
-            s_unit = u;
-            if (u < W.unit_end) {
-                while (q + 1 < W.q_end && p.q[q + 1].unit0 <= u) q++;
-                s_q = q;
-            }
+            s_unit = W.unit_begin + (int64_t)atomicAdd(p.ctr + 3 * wave, 1ull);
         }
         __syncthreads();
         const int64_t u = uniform64(s_unit);
         if (u >= W.unit_end) break;  // every workgroup ends here once the counter passes the wave
-        q = (int32_t)uniform64(s_q);
+        const BUnit U = p.units[u];
+        const int32_t q = (int32_t)uniform64(U.q);
+        const int64_t tile = uniform64(U.tile);
         const BQuery Q = p.q[q];
-        const int64_t r_begin = Q.row0 + (u - Q.unit0) * (int64_t)p.tile_rows;
-        const int64_t r_end = min(r_begin + (int64_t)p.tile_rows, Q.row0 + Q.rows);
-        prev_q = -1;
+        const int64_t k = tile - Q.tile0;  // the query's unit index
+        const int64_t r_begin = max(Q.row0, tile * (int64_t)p.tile_rows);
+        const int64_t r_end = min((tile + 1) * (int64_t)p.tile_rows, Q.row0 + Q.rows);
+        prev_q = Q.chunks == 0 ? q : -1;  // a large query is done after its pass-2 chunks
+        prev_units = (uint32_t)Q.units;
         if (Q.chunks == 0) {
             // small query: dense LDS table of key_mod groups
             int64_t* lsum = (int64_t*)lds;
@@ -264,11 +274,10 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
             __syncthreads();
             for (int g = tid; g < Q.mod; g += bd)
                 if (lcnt[g]) add_global(p, Q.out + g, lcnt[g], lsum[g], lmn[g], lmx[g]);
-            prev_q = q;
-            prev_units = (uint32_t)Q.units;
         } else {
             // large query, pass 1: rows per chunk, one slice of the pool, rows written by chunk
             const int nch = Q.chunks;
+            const int cbits = Q.cbits;
             uint32_t* hist = (uint32_t*)lds;
             uint32_t* cur = hist + nch;
             for (int c = tid; c < nch; c += bd) hist[c] = 0;
@@ -282,7 +291,7 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
             for (int c = c0; c < c1; c++) run += hist[c];
             uint32_t total;
             uint32_t off = block_scan(run, wsum, &total);
-            int32_t* d = p.desc + Q.desc + (u - Q.unit0) * (int64_t)(nch + 1);
+            int32_t* d = p.desc + Q.desc + k * (int64_t)(nch + 1);
             for (int c = c0; c < c1; c++) {
                 cur[c] = off;
                 d[c] = (int32_t)off;
@@ -292,7 +301,7 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
                 d[nch] = (int32_t)total;
                 const int64_t base = total ? (int64_t)atomicAdd(p.ctr + 3 * wave + 2, (unsigned long long)total) : 0;
                 s_base = (base + (int64_t)total <= h.scratch_cap) ? base : -1;
-                p.segb[Q.segb + (u - Q.unit0)] = s_base;
+                p.segb[Q.segb + k] = s_base;
             }
             __syncthreads();
             const int64_t sbase = uniform64(s_base);
@@ -319,10 +328,9 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass2(BArgs p, int32_t wave
     __shared__ int64_t s_unit;
     __shared__ int32_t s_q;
     const int tid = threadIdx.x, bd = blockDim.x;
-    const BHeader h = *p.hdr;
+    const int lane = tid & 63, wid = tid >> 6, nwv = bd >> 6;
     const BWave W = p.waves[wave];
-    const int cbits = h.chunk_bits;
-    const int S = 1 << cbits;
+    const int S = p.chunk_max;
     int64_t* lsum = (int64_t*)lds;
     uint64_t* lmn = (uint64_t*)(lsum + S);
     uint64_t* lmx = lmn + S;
@@ -347,8 +355,8 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass2(BArgs p, int32_t wave
         const BQuery Q = p.q[q];
         const int j = (int)(c - Q.chunk0);
         const int nch = Q.chunks;
-        const int32_t key0 = j << cbits;
-        const int32_t width = min(S, Q.mod - key0);
+        const int32_t key0 = j << Q.cbits;
+        const int32_t width = min(1 << Q.cbits, Q.mod - key0);
         for (int g = tid; g < width; g += bd) {
             lsum[g] = 0;
             lmn[g] = ~0ull;
@@ -356,13 +364,13 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass2(BArgs p, int32_t wave
             lcnt[g] = 0;
         }
         __syncthreads();
-        // every thread walks the chunk's rows of its own units
-        for (int k = tid; k < Q.units; k += bd) {
+        // wavefront w takes units w, w + nwv, ...; its lanes stride over the unit's segment
+        for (int k = wid; k < Q.units; k += nwv) {
             const int64_t base = p.segb[Q.segb + k];
             if (base < 0) continue;  // that unit's rows took the atomic path
             const int32_t* d = p.desc + Q.desc + (int64_t)k * (nch + 1);
             const int32_t lo = d[j], hi = d[j + 1];
-            for (int32_t i = lo; i < hi; i++) {
+            for (int32_t i = lo + lane; i < hi; i += 64) {
                 const uint32_t g = p.sk[base + i] - (uint32_t)key0;
                 const int64_t val = p.sv[base + i];
                 const uint64_t e = enc_i64(val);
@@ -398,11 +406,26 @@ __global__ void __launch_bounds__(256) k_key_column(const int32_t* __restrict__ 
 
 uint64_t magic_of(int32_t mod) { return ~0ull / (uint64_t)(uint32_t)mod + 1; }
 
-// Key chunk of the two-pass path: the largest power of two <= max(lds_groups, 1024).
+// Widest key chunk of the two-pass path: the largest power of two <= max(lds_groups, 1024).
 int chunk_bits_of(int32_t lds_groups) {
     int b = 10;
     while (b < kMaxKeyBits && (1 << (b + 1)) <= lds_groups) b++;
     return b;
+}
+
+// Chunk width of one large query: the widest chunk is narrowed while a chunk would hold more
+// than `target` of the query's estimated rows, down to 64 groups, with at most
+// 2^(24 - widest bits) chunks (the pass-1 histogram).
+int chunk_bits_for(int32_t mod, int64_t est, int cbmax, int64_t target) {
+    const int64_t max_chunks = (int64_t)1 << (kMaxKeyBits - cbmax);
+    int cb = cbmax;
+    while (cb > kMinChunkBits) {
+        if ((double)est * (double)((int64_t)1 << cb) <= (double)target * (double)mod) break;
+        const int nb = cb - 1;
+        if ((((int64_t)mod + (1 << nb) - 1) >> nb) > max_chunks) break;
+        cb = nb;
+    }
+    return cb;
 }
 
 size_t pass1_lds_bytes(const qk_batch_tuning& t) {
@@ -424,14 +447,15 @@ struct qk_batch_s {
     qk_batch_tuning tune{};
     int64_t scratch_rows = (int64_t)1 << 30;
     // device buffers (grown as needed)
-    void* plan = nullptr;  // BHeader + BWave[] + BQuery[]
-    size_t plan_cap = 0;
+    void* plan = nullptr;  // BHeader, BWave[wave_cap], BQuery[q_cap]
     uint32_t* done = nullptr;
     unsigned long long* comp = nullptr;
     int64_t* est = nullptr;
     int32_t q_cap = 0;
     unsigned long long* ctr = nullptr;  // [0] start clock, then 3 per wave
     int32_t wave_cap = 0;
+    BUnit* units = nullptr;
+    int64_t unit_cap = 0;
     int32_t* desc = nullptr;
     int64_t desc_cap = 0;
     int64_t* segb = nullptr;
@@ -441,8 +465,10 @@ struct qk_batch_s {
     int64_t sk_cap = 0;
     unsigned long long *cnt = nullptr, *sum = nullptr, *mn = nullptr, *mx = nullptr;
     int64_t out_cap = 0;
-    void* host_plan = nullptr;  // pinned copy of the plan
-    size_t host_plan_cap = 0;
+    // pinned copies of the plan and the unit list
+    void* host_plan = nullptr;
+    BUnit* host_units = nullptr;
+    int64_t host_unit_cap = 0;
     std::vector<int64_t> offsets;
     std::vector<int64_t> est_host;
     int32_t last_nq = 0;
@@ -455,6 +481,10 @@ struct qk_batch_s {
 namespace qk {
 namespace {
 
+size_t plan_bytes(int32_t wave_cap, int32_t q_cap) {
+    return sizeof(BHeader) + sizeof(BWave) * (size_t)wave_cap + sizeof(BQuery) * (size_t)q_cap;
+}
+
 void free_graph(qk_batch_s* b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
@@ -464,18 +494,17 @@ void free_graph(qk_batch_s* b) {
 template <typename T>
 int grow(T** p, int64_t* cap, int64_t need, int64_t min_cap) {
     if (*p && *cap >= need) return QK_SUCCESS;
+    const int64_t c = std::max(need, std::max(min_cap, *cap + *cap / 2));
     if (*p) pool_free(*p);
     *p = nullptr;
     *cap = 0;
-    const int64_t c = std::max(need, min_cap);
     QK_TRY(pool_alloc((void**)p, (size_t)c * sizeof(T)));
     *cap = c;
     return QK_SUCCESS;
 }
 
-void free_buffers(qk_batch_s* b) {
-    void* ptrs[] = {b->plan, b->done, b->comp, b->est, b->ctr, b->desc, b->segb, b->sk, b->sv,
-                    b->cnt, b->sum, b->mn, b->mx};
+void free_plan(qk_batch_s* b) {
+    void* ptrs[] = {b->plan, b->done, b->comp, b->est, b->ctr};
     for (void* p : ptrs)
         if (p) pool_free(p);
     b->plan = nullptr;
@@ -483,21 +512,30 @@ void free_buffers(qk_batch_s* b) {
     b->comp = nullptr;
     b->est = nullptr;
     b->ctr = nullptr;
+    if (b->host_plan) (void)hipHostFree(b->host_plan);
+    b->host_plan = nullptr;
+    b->q_cap = b->wave_cap = 0;
+}
+
+void free_buffers(qk_batch_s* b) {
+    free_plan(b);
+    void* ptrs[] = {b->units, b->desc, b->segb, b->sk, b->sv, b->cnt, b->sum, b->mn, b->mx};
+    for (void* p : ptrs)
+        if (p) pool_free(p);
+    b->units = nullptr;
     b->desc = nullptr;
     b->segb = nullptr;
     b->sk = nullptr;
     b->sv = nullptr;
     b->cnt = b->sum = b->mn = b->mx = nullptr;
-    b->plan_cap = 0;
-    b->q_cap = b->wave_cap = 0;
-    b->desc_cap = b->segb_cap = b->sk_cap = b->out_cap = 0;
-    if (b->host_plan) (void)hipHostFree(b->host_plan);
-    b->host_plan = nullptr;
-    b->host_plan_cap = 0;
+    b->unit_cap = b->desc_cap = b->segb_cap = b->sk_cap = b->out_cap = 0;
+    if (b->host_units) (void)hipHostFree(b->host_units);
+    b->host_units = nullptr;
+    b->host_unit_cap = 0;
     free_graph(b);
 }
 
-BArgs args_of(const qk_batch_s* b, int32_t nwaves) {
+BArgs args_of(const qk_batch_s* b) {
     BArgs p;
     p.a = b->cols.a;
     p.b = b->cols.b;
@@ -506,9 +544,11 @@ BArgs args_of(const qk_batch_s* b, int32_t nwaves) {
     p.b_range = b->cols.b_range;
     p.tile_rows = b->tune.tile_rows;
     p.lds_groups = b->tune.lds_groups;
+    p.chunk_max = 1 << chunk_bits_of(b->tune.lds_groups);
     p.hdr = (const BHeader*)b->plan;
     p.waves = (const BWave*)((const char*)b->plan + sizeof(BHeader));
-    p.q = (const BQuery*)((const char*)b->plan + sizeof(BHeader) + sizeof(BWave) * (size_t)nwaves);
+    p.q = (const BQuery*)((const char*)b->plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
+    p.units = b->units;
     p.t0 = b->ctr;
     p.ctr = b->ctr + 1;
     p.done = b->done;
@@ -526,7 +566,7 @@ BArgs args_of(const qk_batch_s* b, int32_t nwaves) {
 }
 
 int launch_all(qk_batch_s* b, int32_t nwaves, hipStream_t s) {
-    const BArgs p = args_of(b, nwaves);
+    const BArgs p = args_of(b);
     const int cus = device_info().cus;
     k_batch_init<<<cus * 8, 256, 0, s>>>(p);
     QK_HIP(hipGetLastError());
@@ -578,53 +618,88 @@ int check_tuning(const qk_batch_tuning& t) {
     return QK_SUCCESS;
 }
 
-int ensure_plan(qk_batch_s* b, size_t bytes, int32_t nq, int32_t nwaves) {
-    if (bytes > b->plan_cap || nq > b->q_cap || nwaves > b->wave_cap) {
-        free_graph(b);
-        void* ptrs[] = {b->plan, b->done, b->comp, b->est, b->ctr};
-        for (void* p : ptrs)
-            if (p) pool_free(p);
-        b->plan = nullptr;
-        b->done = nullptr;
-        b->comp = nullptr;
-        b->est = nullptr;
-        b->ctr = nullptr;
-        if (b->host_plan) (void)hipHostFree(b->host_plan);
+// Plan buffers for nq queries in nwaves waves (capacities only grow; the graph keeps pointers).
+int ensure_plan(qk_batch_s* b, int32_t nq, int32_t nwaves) {
+    if (b->plan && nq <= b->q_cap && nwaves <= b->wave_cap) return QK_SUCCESS;
+    free_graph(b);
+    const int32_t qc = std::max(nq, std::max(b->q_cap * 2, 1024));
+    const int32_t wc = std::max(nwaves, std::max(b->wave_cap * 2, 64));
+    free_plan(b);
+    const size_t pc = plan_bytes(wc, qc);
+    QK_TRY(pool_alloc(&b->plan, pc));
+    QK_TRY(pool_alloc((void**)&b->done, (size_t)qc * 4));
+    QK_TRY(pool_alloc((void**)&b->comp, (size_t)qc * 8));
+    QK_TRY(pool_alloc((void**)&b->est, (size_t)qc * 8));
+    QK_TRY(pool_alloc((void**)&b->ctr, (size_t)(1 + 3 * wc) * 8));
+    if (hipHostMalloc(&b->host_plan, pc) != hipSuccess) {
         b->host_plan = nullptr;
-        const size_t pc = std::max(bytes, b->plan_cap * 2);
-        const int32_t qc = std::max(nq, std::max(b->q_cap * 2, 1024));
-        const int32_t wc = std::max(nwaves, std::max(b->wave_cap * 2, 64));
-        b->plan_cap = b->host_plan_cap = 0;
-        b->q_cap = b->wave_cap = 0;
-        QK_TRY(pool_alloc(&b->plan, pc));
-        QK_TRY(pool_alloc((void**)&b->done, (size_t)qc * 4));
-        QK_TRY(pool_alloc((void**)&b->comp, (size_t)qc * 8));
-        QK_TRY(pool_alloc((void**)&b->est, (size_t)qc * 8));
-        QK_TRY(pool_alloc((void**)&b->ctr, (size_t)(1 + 3 * wc) * 8));
-        if (hipHostMalloc(&b->host_plan, pc) != hipSuccess) {
-            b->host_plan = nullptr;
-            return fail(QK_OUT_OF_MEMORY, "qk_batch: pinned plan buffer");
-        }
-        b->plan_cap = b->host_plan_cap = pc;
-        b->q_cap = qc;
-        b->wave_cap = wc;
+        return fail(QK_OUT_OF_MEMORY, "qk_batch: pinned plan buffer");
     }
+    b->q_cap = qc;
+    b->wave_cap = wc;
     return QK_SUCCESS;
+}
+
+int ensure_units(qk_batch_s* b, int64_t n) {
+    if (b->units && b->host_units && n <= b->unit_cap && n <= b->host_unit_cap) return QK_SUCCESS;
+    free_graph(b);
+    const int64_t c = std::max(n, std::max(b->unit_cap * 2, (int64_t)1 << 16));
+    if (b->units) pool_free(b->units);
+    b->units = nullptr;
+    b->unit_cap = 0;
+    if (b->host_units) (void)hipHostFree(b->host_units);
+    b->host_units = nullptr;
+    b->host_unit_cap = 0;
+    QK_TRY(pool_alloc((void**)&b->units, (size_t)c * sizeof(BUnit)));
+    if (hipHostMalloc((void**)&b->host_units, (size_t)c * sizeof(BUnit)) != hipSuccess) {
+        b->host_units = nullptr;
+        return fail(QK_OUT_OF_MEMORY, "qk_batch: pinned unit buffer");
+    }
+    b->unit_cap = b->host_unit_cap = c;
+    return QK_SUCCESS;
+}
+
+// Units of queries [q_begin, q_end) of the plan, tile by tile (queries of one tile in plan order).
+void list_units(const std::vector<BQuery>& bq, int32_t q_begin, int32_t q_end, BUnit* out, int64_t* n) {
+    if (q_begin >= q_end) return;
+    int64_t t_lo = INT64_MAX, t_hi = 0;
+    for (int32_t k = q_begin; k < q_end; k++) {
+        t_lo = std::min(t_lo, bq[(size_t)k].tile0);
+        t_hi = std::max(t_hi, bq[(size_t)k].tile0 + bq[(size_t)k].units);
+    }
+    std::vector<int64_t> at((size_t)(t_hi - t_lo + 1), 0);
+    for (int32_t k = q_begin; k < q_end; k++) {
+        at[(size_t)(bq[(size_t)k].tile0 - t_lo)] += 1;
+        at[(size_t)(bq[(size_t)k].tile0 + bq[(size_t)k].units - t_lo)] -= 1;
+    }
+    // at[t] = queries covering tile t, then its first slot
+    int64_t run = 0, pos = *n;
+    for (size_t t = 0; t + 1 < at.size(); t++) {
+        run += at[t];
+        at[t] = pos;
+        pos += run;
+    }
+    for (int32_t k = q_begin; k < q_end; k++) {
+        const BQuery& d = bq[(size_t)k];
+        for (int64_t t = d.tile0; t < d.tile0 + d.units; t++) out[at[(size_t)(t - t_lo)]++] = BUnit{k, (int32_t)t};
+    }
+    *n = pos;
 }
 
 int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s) {
     const qk_batch_columns& c = b->cols;
     const qk_batch_tuning& t = b->tune;
     const int64_t nblocks = (c.rows + c.block_rows - 1) / c.block_rows;
-    const int cbits = chunk_bits_of(t.lds_groups);
+    const int64_t T = t.tile_rows;
+    const int cbmax = chunk_bits_of(t.lds_groups);
     if (nq > (1 << 24)) return fail(QK_ERROR, "qk_batch: at most 2^24 queries per run");
+    if ((c.rows + T - 1) / T > INT32_MAX) return fail(QK_ERROR, "qk_batch: too many tiles; raise tile_rows");
     // queries in plan order: small ones first, then large ones (waves are cut below)
     std::vector<int32_t> order;
     order.reserve((size_t)nq);
     for (int pass = 0; pass < 2; pass++)
         for (int32_t i = 0; i < nq; i++)
             if ((q[i].key_mod > t.lds_groups) == (pass == 1)) order.push_back(i);
-    std::vector<BQuery> bq((size_t)nq);
     b->offsets.assign((size_t)nq + 1, 0);
     int64_t out = 0;
     for (int32_t i = 0; i < nq; i++) {
@@ -637,7 +712,7 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         out += x.key_mod;
     }
     b->offsets[(size_t)nq] = out;
-    int64_t units = 0, desc = 0, segb = 0, chunk_units = 0;
+    std::vector<BQuery> bq((size_t)nq);
     int32_t nlarge = 0;
     for (size_t k = 0; k < order.size(); k++) {
         const int32_t i = order[k];
@@ -646,28 +721,20 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         d = BQuery{};
         d.row0 = (int64_t)x.first_block * c.block_rows;
         d.rows = std::min<int64_t>((int64_t)(x.first_block + x.nblocks) * c.block_rows, c.rows) - d.row0;
-        d.unit0 = units;
-        d.units = (int32_t)((d.rows + t.tile_rows - 1) / t.tile_rows);
+        d.tile0 = d.row0 / T;
+        d.units = (int32_t)((d.row0 + d.rows + T - 1) / T - d.tile0);
         d.ts_lo = x.ts_lo;
         d.ts_hi = x.ts_hi;
         d.out = b->offsets[(size_t)i];
         d.mod = x.key_mod;
         d.magic = magic_of(x.key_mod);
         d.index = i;
-        if (x.key_mod > t.lds_groups) {
-            d.chunks = (int32_t)(((int64_t)x.key_mod + (1 << cbits) - 1) >> cbits);
-            d.desc = desc;
-            d.segb = segb;
-            d.chunk0 = chunk_units;
-            desc += (int64_t)d.units * (d.chunks + 1);
-            segb += d.units;
-            chunk_units += d.chunks;
-            nlarge++;
-        }
-        units += d.units;
+        nlarge += x.key_mod > t.lds_groups;
     }
+    const int32_t nsmall = nq - nlarge;
     // grow the buffers that depend on the batch
     if (out > b->out_cap) {
+        free_graph(b);
         unsigned long long** arrs[] = {&b->cnt, &b->sum, &b->mn, &b->mx};
         const int64_t cap = std::max(out, std::max(b->out_cap + b->out_cap / 2, (int64_t)1 << 20));
         for (unsigned long long** a : arrs) {
@@ -678,10 +745,9 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         for (unsigned long long** a : arrs) QK_TRY(pool_alloc((void**)a, (size_t)cap * 8));
         b->out_cap = cap;
     }
-    QK_TRY(grow(&b->desc, &b->desc_cap, std::max<int64_t>(desc, 1), (int64_t)1 << 16));
-    QK_TRY(grow(&b->segb, &b->segb_cap, std::max<int64_t>(segb, 1), (int64_t)1 << 12));
     const int64_t pool = b->scratch_rows;
     if (nlarge && (!b->sk || !b->sv || b->sk_cap < pool)) {
+        free_graph(b);
         if (b->sk) pool_free(b->sk);
         if (b->sv) pool_free(b->sv);
         b->sk = nullptr;
@@ -691,74 +757,91 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         QK_TRY(pool_alloc((void**)&b->sv, (size_t)pool * 8));
         b->sk_cap = pool;
     }
-    const size_t max_plan = sizeof(BHeader) + sizeof(BWave) * (size_t)std::max(1, nlarge) + sizeof(BQuery) * (size_t)std::max(nq, 1);
-    QK_TRY(ensure_plan(b, max_plan, nq, std::max(1, nlarge)));
-    // estimate the passing rows of the large queries (sampling) to cut the waves
-    const int32_t nsmall = nq - nlarge;
-    std::vector<BWave> waves;
-    BWave w{};
-    w.q_begin = 0;
-    w.unit_begin = 0;
-    w.chunk_begin = 0;
-    QK_HIP(hipStreamSynchronize(s));  // the pinned plan is rewritten below
+    QK_TRY(ensure_plan(b, nq, std::max(1, nlarge)));
+    QK_HIP(hipStreamSynchronize(s));  // the pinned plan and unit list are rewritten below
+    BQuery* host_q = (BQuery*)((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
+    const BArgs p0 = args_of(b);
+    // estimate the passing rows of the large queries (sampling) to size chunks and cut waves
+    b->est_host.assign((size_t)nlarge, 0);
     if (nlarge) {
-        BHeader hd{};
-        hd.nq = nq;
-        memcpy(b->host_plan, &hd, sizeof hd);
-        // estimate layout: header, one wave slot, queries
-        memcpy((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave), bq.data(), sizeof(BQuery) * (size_t)nq);
-        QK_HIP(hipMemcpyAsync(b->plan, b->host_plan, sizeof(BHeader) + sizeof(BWave) + sizeof(BQuery) * (size_t)nq,
+        memcpy(host_q + nsmall, bq.data() + nsmall, sizeof(BQuery) * (size_t)nlarge);
+        QK_HIP(hipMemcpyAsync((void*)(p0.q + nsmall), host_q + nsmall, sizeof(BQuery) * (size_t)nlarge,
                               hipMemcpyHostToDevice, s));
-        BArgs ep = args_of(b, 1);
-        ep.q += nsmall;  // the large queries
+        BArgs ep = p0;
+        ep.q += nsmall;
         k_batch_estimate<<<nlarge, 256, 0, s>>>(ep, nlarge);
         QK_HIP(hipGetLastError());
-        b->est_host.assign((size_t)nlarge, 0);
         QK_HIP(hipMemcpyAsync(b->est_host.data(), b->est, (size_t)nlarge * 8, hipMemcpyDeviceToHost, s));
         QK_HIP(hipStreamSynchronize(s));
     }
+    // chunks of the large queries
+    const int64_t target = std::max<int64_t>((int64_t)1 << 18, pool / ((int64_t)device_info().cus * 4));
+    int64_t desc = 0, segb = 0, chunk_units = 0;
+    for (int32_t k = nsmall; k < nq; k++) {
+        BQuery& d = bq[(size_t)k];
+        d.cbits = chunk_bits_for(d.mod, b->est_host[(size_t)(k - nsmall)], cbmax, target);
+        d.chunks = (int32_t)(((int64_t)d.mod + (1 << d.cbits) - 1) >> d.cbits);
+        d.desc = desc;
+        d.segb = segb;
+        d.chunk0 = chunk_units;
+        desc += (int64_t)d.units * (d.chunks + 1);
+        segb += d.units;
+        chunk_units += d.chunks;
+    }
+    QK_TRY(grow(&b->desc, &b->desc_cap, std::max<int64_t>(desc, 1), (int64_t)1 << 16));
+    QK_TRY(grow(&b->segb, &b->segb_cap, std::max<int64_t>(segb, 1), (int64_t)1 << 12));
     // waves: wave 0 holds every small query; large queries are added while their estimated
     // rows fit the pool (a query above the pool gets a wave of its own)
+    std::vector<BWave> waves;
+    BWave w{};
     int64_t used = 0;
     w.q_end = nsmall;
-    w.unit_end = nsmall ? bq[(size_t)nsmall - 1].unit0 + bq[(size_t)nsmall - 1].units : 0;
-    w.chunk_end = 0;
     for (int32_t k = nsmall; k < nq; k++) {
         const BQuery& d = bq[(size_t)k];
-        const int64_t need = b->est_host[(size_t)(k - nsmall)] + b->est_host[(size_t)(k - nsmall)] / 4 + 4096;
+        const int64_t e = b->est_host[(size_t)(k - nsmall)];
+        const int64_t need = e + e / 4 + 4096;
         if (used > 0 && used + need > pool) {
             waves.push_back(w);
             w = BWave{};
             w.q_begin = k;
-            w.unit_begin = d.unit0;
             w.chunk_begin = d.chunk0;
             used = 0;
         }
         used += need;
         w.q_end = k + 1;
-        w.unit_end = d.unit0 + d.units;
         w.chunk_end = d.chunk0 + d.chunks;
     }
     waves.push_back(w);
     const int32_t nw = (int32_t)waves.size();
+    int64_t nunits = 0;
+    for (const BQuery& d : bq) nunits += d.units;
+    QK_TRY(ensure_units(b, std::max<int64_t>(nunits, 1)));
+    QK_TRY(ensure_plan(b, nq, nw));
+    int64_t pos = 0;
+    for (BWave& x : waves) {
+        x.unit_begin = pos;
+        list_units(bq, x.q_begin, x.q_end, b->host_units, &pos);
+        x.unit_end = pos;
+    }
     BHeader hd{};
     hd.nq = nq;
     hd.nwaves = nw;
     hd.out_total = out;
     hd.scratch_cap = nlarge ? pool : 0;
-    hd.chunk_bits = cbits;
-    const size_t plan_bytes = sizeof hd + sizeof(BWave) * (size_t)nw + sizeof(BQuery) * (size_t)nq;
-    QK_TRY(ensure_plan(b, plan_bytes, nq, nw));
+    hd.nunits = nunits;
+    host_q = (BQuery*)((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
     memcpy(b->host_plan, &hd, sizeof hd);
     memcpy((char*)b->host_plan + sizeof hd, waves.data(), sizeof(BWave) * (size_t)nw);
-    if (nq) memcpy((char*)b->host_plan + sizeof hd + sizeof(BWave) * (size_t)nw, bq.data(), sizeof(BQuery) * (size_t)nq);
-    QK_HIP(hipMemcpyAsync(b->plan, b->host_plan, plan_bytes, hipMemcpyHostToDevice, s));
+    if (nq) memcpy(host_q, bq.data(), sizeof(BQuery) * (size_t)nq);
+    const BArgs p = args_of(b);
+    QK_HIP(hipMemcpyAsync(b->plan, b->host_plan, sizeof hd + sizeof(BWave) * (size_t)nw, hipMemcpyHostToDevice, s));
+    if (nq) QK_HIP(hipMemcpyAsync((void*)p.q, host_q, sizeof(BQuery) * (size_t)nq, hipMemcpyHostToDevice, s));
+    if (nunits) QK_HIP(hipMemcpyAsync(b->units, b->host_units, sizeof(BUnit) * (size_t)nunits, hipMemcpyHostToDevice, s));
     b->last_nq = nq;
     if (t.use_graph) {
-        const BArgs now = args_of(b, nw);
-        if (b->graph_waves != nw || memcmp(&now, &b->graph_args, sizeof now) != 0) {
+        if (b->graph_waves != nw || memcmp(&p, &b->graph_args, sizeof p) != 0) {
             build_graph(b, nw);
-            b->graph_args = now;
+            b->graph_args = p;
         }
         if (b->graph_waves == nw) {
             QK_HIP(hipGraphLaunch(b->graph, s));
