@@ -305,7 +305,8 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
  *
  * Many independent `SELECT key, SUM(v), MIN(v), MAX(v), COUNT(v) FROM t WHERE lo <= ts < hi
  * GROUP BY key` queries over one device-resident table, run together by one persistent
- * kernel that takes (query, tile) work units from a device queue.  Each query reads a
+ * kernel that takes (tile, query) work units from device queues, one queue per die of the
+ * device (the queries of one tile run on one die and share its L2).  Each query reads a
  * contiguous range of the table's blocks and groups by a computed key
  *     key = (a * b_range + b) mod key_mod        (Euclidean remainder, 0 <= key < key_mod)
  * of two int32 columns, so its groups are the dense range [0, key_mod).  A query with at
@@ -334,11 +335,13 @@ typedef struct qk_batch_query {
 } qk_batch_query;
 
 typedef struct qk_batch_tuning {
-    int32_t tile_rows;      /* rows per work unit (multiple of 4096) */
+    int32_t tile_rows;      /* rows per work unit (multiple of 4096; its pass bits, tile_rows / 8
+                               bytes, and the chunk histogram must fit the LDS) */
     int32_t lds_groups;     /* largest key_mod aggregated in LDS in one pass */
     int32_t wgs_per_cu;     /* persistent workgroups per CU (1 or 2) */
     int32_t use_graph;      /* replay the batch as a captured hipGraph */
-    int64_t scratch_rows;   /* rows of the pool of the two-pass path (0 = 2^30, 12 B per row) */
+    int64_t scratch_rows;   /* most rows of the pool of the two-pass path, 12 B per row (0 = 2^32);
+                               it is allocated as large as a batch's estimated rows need */
 } qk_batch_tuning;
 
 typedef struct qk_batch_s* qk_batch;

@@ -4,7 +4,7 @@ include/qikkdb_amd.h).
 Each query is ``SELECT key, SUM(value), MIN(value), MAX(value), COUNT(value) FROM t
 WHERE ts_lo <= ts < ts_hi GROUP BY key`` over a contiguous range of the table's blocks, with
 ``key = (a * b_range + b) mod key_mod``.  All queries of a run are executed by one
-persistent kernel that takes (query, tile) work units from a device queue.
+persistent kernel that takes (tile, query) work units from device queues, one per die.
 """
 from __future__ import annotations
 
@@ -36,7 +36,7 @@ class Query:
 
 @dataclass
 class Tuning:
-    tile_rows: int = 1 << 20
+    tile_rows: int = 1 << 18
     lds_groups: int = 4096
     wgs_per_cu: int = 1
     use_graph: int = 1

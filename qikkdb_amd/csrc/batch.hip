@@ -39,6 +39,11 @@ constexpr int kBRows = 4;            // rows per thread per step of a unit
 constexpr int kSamples = 4096;       // rows sampled per large query for its estimate
 constexpr int kMaxKeyBits = 24;      // key_mod <= 2^24
 constexpr int kMinChunkBits = 6;     // narrowest key chunk of a large query: 64 groups
+constexpr int64_t kDefaultScratchRows = (int64_t)1 << 32;  // 48 GiB at most, allocated as needed
+constexpr int kXcds = 8;             // dies of the device, each with its own L2
+constexpr int kCtrStride = 16;       // counters per wave: kXcds unit lists, pass-2 units, pool top
+constexpr int kCtrPass2 = kXcds;
+constexpr int kCtrPool = kXcds + 1;
 
 struct BQuery {
     int64_t row0, rows;   // table rows of the query
@@ -63,7 +68,7 @@ struct BUnit {
 };
 
 struct BWave {
-    int64_t unit_begin, unit_end;    // pass-1 units
+    int64_t unit_off[kXcds + 1];     // pass-1 units of list x: [unit_off[x], unit_off[x + 1])
     int64_t chunk_begin, chunk_end;  // pass-2 units
     int32_t q_begin, q_end;          // queries of the wave (plan order)
 };
@@ -88,7 +93,7 @@ struct BArgs {
     const BWave* waves;
     const BQuery* q;
     const BUnit* units;
-    unsigned long long* ctr;    // per wave: [3w] pass-1 units, [3w+1] pass-2 units, [3w+2] pool top
+    unsigned long long* ctr;    // kCtrStride per wave
     unsigned long long* t0;     // start clock
     uint32_t* done;             // [nq] finished units (pass 1 for small, pass 2 for large queries)
     unsigned long long* comp;   // [nq, caller order] completion clock
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(256) k_batch_init(BArgs p) {
         p.done[i] = 0;
         p.comp[i] = 0;
     }
-    for (uint64_t i = i0; i < 3 * (uint64_t)h.nwaves; i += stride) p.ctr[i] = 0;
+    for (uint64_t i = i0; i < (uint64_t)kCtrStride * h.nwaves; i += stride) p.ctr[i] = 0;
     if (i0 == 0) *p.t0 = wall_clock64();
 }
 
@@ -223,24 +228,43 @@ __device__ __forceinline__ void for_passing_rows(const BArgs& p, const BQuery& Q
     }
 }
 
+// Claims the next pass-1 unit of a wave: first from the list of this workgroup's die (units
+// of one tile share that die's L2), then from the other lists once it is empty.  -1 when
+// every list is empty.  `xi` is the caller's position in that order.
+__device__ __forceinline__ int64_t claim_unit(const BArgs& p, const BWave& W, int32_t wave, int& xi) {
+    const int home = blockIdx.x % kXcds;
+    while (xi < kXcds) {
+        const int x = (home + xi) % kXcds;
+        const int64_t n = W.unit_off[x + 1] - W.unit_off[x];
+        if (n > 0) {
+            const int64_t c = (int64_t)atomicAdd(p.ctr + kCtrStride * wave + x, 1ull);
+            if (c < n) return W.unit_off[x] + c;
+        }
+        xi++;
+    }
+    return -1;
+}
+
 __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave) {
     extern __shared__ __align__(16) uint64_t lds[];
     __shared__ int64_t s_unit, s_base;
     __shared__ uint32_t wsum[16];
     const int tid = threadIdx.x, bd = blockDim.x;
+    const int lane = tid & 63, wid = tid >> 6, nwv = bd >> 6;
     const BHeader h = *p.hdr;
     const BWave W = p.waves[wave];
     const int G = p.lds_groups;
     int32_t prev_q = -1;
     uint32_t prev_units = 0;
+    int xi = 0;
     while (true) {
         if (tid == 0) {
             if (prev_q >= 0) mark_done(p, prev_q, prev_units);
-            s_unit = W.unit_begin + (int64_t)atomicAdd(p.ctr + 3 * wave, 1ull);
+            s_unit = claim_unit(p, W, wave, xi);
         }
         __syncthreads();
         const int64_t u = uniform64(s_unit);
-        if (u >= W.unit_end) break;  // every workgroup ends here once the counter passes the wave
+        if (u < 0) break;  // every workgroup ends here once all lists of the wave are empty
         const BUnit U = p.units[u];
         const int32_t q = (int32_t)uniform64(U.q);
         const int64_t tile = uniform64(U.tile);
@@ -275,14 +299,30 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
             for (int g = tid; g < Q.mod; g += bd)
                 if (lcnt[g]) add_global(p, Q.out + g, lcnt[g], lsum[g], lmn[g], lmx[g]);
         } else {
-            // large query, pass 1: rows per chunk, one slice of the pool, rows written by chunk
+            // large query, pass 1: rows per chunk, one slice of the pool, rows written by chunk.
+            // The first sweep keeps one pass bit per row in LDS (a wavefront ballot per 64
+            // rows), so the second sweep does not read ts again.
             const int nch = Q.chunks;
             const int cbits = Q.cbits;
-            uint32_t* hist = (uint32_t*)lds;
+            uint64_t* mask = lds;  // tile_rows / 64 words
+            uint32_t* hist = (uint32_t*)(mask + (p.tile_rows >> 6));
             uint32_t* cur = hist + nch;
             for (int c = tid; c < nch; c += bd) hist[c] = 0;
             __syncthreads();
-            for_passing_rows(p, Q, r_begin, r_end, [&](int64_t, uint32_t key) { atomicAdd(&hist[key >> cbits], 1u); });
+            int word = wid;
+            for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
+                int64_t tsv[kBRows];
+#pragma unroll
+                for (int j = 0; j < kBRows; j++) tsv[j] = p.ts[min(base + (int64_t)j * bd + tid, r_end - 1)];
+#pragma unroll
+                for (int j = 0; j < kBRows; j++, word += nwv) {
+                    const int64_t r = base + (int64_t)j * bd + tid;
+                    const bool pass = r < r_end && tsv[j] >= Q.ts_lo && tsv[j] < Q.ts_hi;
+                    const uint64_t m = __ballot(pass);
+                    if (lane == 0) mask[word] = m;
+                    if (pass) atomicAdd(&hist[key_of(p.a[r], p.b[r], p.b_range, Q) >> cbits], 1u);
+                }
+            }
             __syncthreads();
             // exclusive offsets: thread t owns chunks [t * per, (t + 1) * per)
             const int per = (nch + bd - 1) / bd;
@@ -299,23 +339,31 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
             }
             if (tid == 0) {
                 d[nch] = (int32_t)total;
-                const int64_t base = total ? (int64_t)atomicAdd(p.ctr + 3 * wave + 2, (unsigned long long)total) : 0;
+                const int64_t base =
+                    total ? (int64_t)atomicAdd(p.ctr + kCtrStride * wave + kCtrPool, (unsigned long long)total) : 0;
                 s_base = (base + (int64_t)total <= h.scratch_cap) ? base : -1;
                 p.segb[Q.segb + k] = s_base;
             }
             __syncthreads();
             const int64_t sbase = uniform64(s_base);
-            for_passing_rows(p, Q, r_begin, r_end, [&](int64_t r, uint32_t key) {
-                const int64_t val = p.v[r];
-                if (sbase >= 0) {
-                    const uint32_t at = atomicAdd(&cur[key >> cbits], 1u);
-                    p.sk[sbase + at] = key;
-                    p.sv[sbase + at] = val;
-                } else {
-                    const uint64_t e = enc_i64(val);
-                    add_global(p, Q.out + key, 1u, val, e, e);
+            word = wid;
+            for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
+#pragma unroll
+                for (int j = 0; j < kBRows; j++, word += nwv) {
+                    if (!((mask[word] >> lane) & 1)) continue;
+                    const int64_t r = base + (int64_t)j * bd + tid;
+                    const uint32_t key = key_of(p.a[r], p.b[r], p.b_range, Q);
+                    const int64_t val = p.v[r];
+                    if (sbase >= 0) {
+                        const uint32_t at = atomicAdd(&cur[key >> cbits], 1u);
+                        p.sk[sbase + at] = key;
+                        p.sv[sbase + at] = val;
+                    } else {
+                        const uint64_t e = enc_i64(val);
+                        add_global(p, Q.out + key, 1u, val, e, e);
+                    }
                 }
-            });
+            }
         }
         __syncthreads();  // the unit is done (and its LDS free) for every thread
     }
@@ -341,7 +389,7 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass2(BArgs p, int32_t wave
     while (true) {
         if (tid == 0) {
             if (prev_q >= 0) mark_done(p, prev_q, prev_chunks);
-            const int64_t c = W.chunk_begin + (int64_t)atomicAdd(p.ctr + 3 * wave + 1, 1ull);
+            const int64_t c = W.chunk_begin + (int64_t)atomicAdd(p.ctr + kCtrStride * wave + kCtrPass2, 1ull);
             s_unit = c;
             if (c < W.chunk_end) {
                 while (q < W.q_end && (p.q[q].chunks == 0 || p.q[q].chunk0 + p.q[q].chunks <= c)) q++;
@@ -431,7 +479,8 @@ int chunk_bits_for(int32_t mod, int64_t est, int cbmax, int64_t target) {
 size_t pass1_lds_bytes(const qk_batch_tuning& t) {
     const int cb = chunk_bits_of(t.lds_groups);
     const size_t small = (size_t)t.lds_groups * 28;
-    const size_t large = (size_t)8 << (kMaxKeyBits - cb);  // hist + cursors for 2^24 keys
+    // pass bits of a tile, then hist + cursors for 2^24 keys
+    const size_t large = (size_t)t.tile_rows / 8 + ((size_t)8 << (kMaxKeyBits - cb));
     return std::max(small, large) + 64;
 }
 
@@ -445,7 +494,7 @@ using namespace qk;
 struct qk_batch_s {
     qk_batch_columns cols{};
     qk_batch_tuning tune{};
-    int64_t scratch_rows = (int64_t)1 << 30;
+    int64_t scratch_rows = kDefaultScratchRows;
     // device buffers (grown as needed)
     void* plan = nullptr;  // BHeader, BWave[wave_cap], BQuery[q_cap]
     uint32_t* done = nullptr;
@@ -610,11 +659,11 @@ int check_tuning(const qk_batch_tuning& t) {
     if (t.tile_rows < 4096 || t.tile_rows % 4096) return fail(QK_ERROR, "qk_batch: tile_rows must be a positive multiple of 4096");
     const size_t limit = std::min<size_t>(device_info().lds_per_block, 160 * 1024) - 512;
     if (t.lds_groups < 0 || pass1_lds_bytes(t) > limit || pass2_lds_bytes(t) > limit)
-        return fail(QK_ERROR, "qk_batch: lds_groups does not fit the LDS");
+        return fail(QK_ERROR, "qk_batch: tile_rows and lds_groups do not fit the LDS");
     if (t.wgs_per_cu < 1 || t.wgs_per_cu > 2) return fail(QK_ERROR, "qk_batch: wgs_per_cu must be 1 or 2");
     const size_t half = std::min<size_t>(device_info().lds_per_cu, 160 * 1024) / 2 - 512;
     if (t.wgs_per_cu == 2 && (pass1_lds_bytes(t) > half || pass2_lds_bytes(t) > half))
-        return fail(QK_ERROR, "qk_batch: two workgroups per CU need lds_groups below half the LDS");
+        return fail(QK_ERROR, "qk_batch: two workgroups per CU need tile_rows and lds_groups below half the LDS");
     return QK_SUCCESS;
 }
 
@@ -630,7 +679,7 @@ int ensure_plan(qk_batch_s* b, int32_t nq, int32_t nwaves) {
     QK_TRY(pool_alloc((void**)&b->done, (size_t)qc * 4));
     QK_TRY(pool_alloc((void**)&b->comp, (size_t)qc * 8));
     QK_TRY(pool_alloc((void**)&b->est, (size_t)qc * 8));
-    QK_TRY(pool_alloc((void**)&b->ctr, (size_t)(1 + 3 * wc) * 8));
+    QK_TRY(pool_alloc((void**)&b->ctr, (size_t)(1 + kCtrStride * wc) * 8));
     if (hipHostMalloc(&b->host_plan, pc) != hipSuccess) {
         b->host_plan = nullptr;
         return fail(QK_OUT_OF_MEMORY, "qk_batch: pinned plan buffer");
@@ -659,27 +708,34 @@ int ensure_units(qk_batch_s* b, int64_t n) {
     return QK_SUCCESS;
 }
 
-// Units of queries [q_begin, q_end) of the plan, tile by tile (queries of one tile in plan order).
-void list_units(const std::vector<BQuery>& bq, int32_t q_begin, int32_t q_end, BUnit* out, int64_t* n) {
-    if (q_begin >= q_end) return;
+// Units of queries [q_begin, q_end) of the plan, appended at out[*n] as kXcds lists: list x
+// holds the tiles t with t mod kXcds == x, tile by tile (the queries of a tile in plan order).
+void list_units(const std::vector<BQuery>& bq, BWave* w, BUnit* out, int64_t* n) {
     int64_t t_lo = INT64_MAX, t_hi = 0;
-    for (int32_t k = q_begin; k < q_end; k++) {
+    for (int32_t k = w->q_begin; k < w->q_end; k++) {
         t_lo = std::min(t_lo, bq[(size_t)k].tile0);
         t_hi = std::max(t_hi, bq[(size_t)k].tile0 + bq[(size_t)k].units);
     }
+    if (w->q_begin >= w->q_end) t_lo = t_hi = 0;
     std::vector<int64_t> at((size_t)(t_hi - t_lo + 1), 0);
-    for (int32_t k = q_begin; k < q_end; k++) {
+    for (int32_t k = w->q_begin; k < w->q_end; k++) {
         at[(size_t)(bq[(size_t)k].tile0 - t_lo)] += 1;
         at[(size_t)(bq[(size_t)k].tile0 + bq[(size_t)k].units - t_lo)] -= 1;
     }
-    // at[t] = queries covering tile t, then its first slot
-    int64_t run = 0, pos = *n;
-    for (size_t t = 0; t + 1 < at.size(); t++) {
-        run += at[t];
-        at[t] = pos;
-        pos += run;
+    std::vector<int64_t> cover((size_t)(t_hi - t_lo), 0);  // queries covering each tile
+    int64_t run = 0;
+    for (size_t i = 0; i < cover.size(); i++) cover[i] = run += at[i];
+    int64_t pos = *n;
+    for (int x = 0; x < kXcds; x++) {
+        w->unit_off[x] = pos;
+        int64_t t = t_lo + (((int64_t)x - t_lo) % kXcds + kXcds) % kXcds;  // first tile of list x
+        for (; t < t_hi; t += kXcds) {
+            at[(size_t)(t - t_lo)] = pos;
+            pos += cover[(size_t)(t - t_lo)];
+        }
     }
-    for (int32_t k = q_begin; k < q_end; k++) {
+    w->unit_off[kXcds] = pos;
+    for (int32_t k = w->q_begin; k < w->q_end; k++) {
         const BQuery& d = bq[(size_t)k];
         for (int64_t t = d.tile0; t < d.tile0 + d.units; t++) out[at[(size_t)(t - t_lo)]++] = BUnit{k, (int32_t)t};
     }
@@ -745,18 +801,6 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         for (unsigned long long** a : arrs) QK_TRY(pool_alloc((void**)a, (size_t)cap * 8));
         b->out_cap = cap;
     }
-    const int64_t pool = b->scratch_rows;
-    if (nlarge && (!b->sk || !b->sv || b->sk_cap < pool)) {
-        free_graph(b);
-        if (b->sk) pool_free(b->sk);
-        if (b->sv) pool_free(b->sv);
-        b->sk = nullptr;
-        b->sv = nullptr;
-        b->sk_cap = 0;
-        QK_TRY(pool_alloc((void**)&b->sk, (size_t)pool * 4));
-        QK_TRY(pool_alloc((void**)&b->sv, (size_t)pool * 8));
-        b->sk_cap = pool;
-    }
     QK_TRY(ensure_plan(b, nq, std::max(1, nlarge)));
     QK_HIP(hipStreamSynchronize(s));  // the pinned plan and unit list are rewritten below
     BQuery* host_q = (BQuery*)((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
@@ -774,6 +818,23 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         QK_HIP(hipMemcpyAsync(b->est_host.data(), b->est, (size_t)nlarge * 8, hipMemcpyDeviceToHost, s));
         QK_HIP(hipStreamSynchronize(s));
     }
+    // the pool: the batch's estimated rows up to scratch_rows (grown, never shrunk)
+    int64_t need_all = 0;
+    for (int64_t e : b->est_host) need_all += e + e / 4 + 4096;
+    const int64_t want = std::min(b->scratch_rows, need_all);
+    if (nlarge && (!b->sk || !b->sv || b->sk_cap < want)) {
+        free_graph(b);
+        const int64_t cap = std::min(b->scratch_rows, std::max(want, b->sk_cap + b->sk_cap / 2));
+        if (b->sk) pool_free(b->sk);
+        if (b->sv) pool_free(b->sv);
+        b->sk = nullptr;
+        b->sv = nullptr;
+        b->sk_cap = 0;
+        QK_TRY(pool_alloc((void**)&b->sk, (size_t)cap * 4));
+        QK_TRY(pool_alloc((void**)&b->sv, (size_t)cap * 8));
+        b->sk_cap = cap;
+    }
+    const int64_t pool = b->sk_cap;
     // chunks of the large queries
     const int64_t target = std::max<int64_t>((int64_t)1 << 18, pool / ((int64_t)device_info().cus * 4));
     int64_t desc = 0, segb = 0, chunk_units = 0;
@@ -818,11 +879,7 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
     QK_TRY(ensure_units(b, std::max<int64_t>(nunits, 1)));
     QK_TRY(ensure_plan(b, nq, nw));
     int64_t pos = 0;
-    for (BWave& x : waves) {
-        x.unit_begin = pos;
-        list_units(bq, x.q_begin, x.q_end, b->host_units, &pos);
-        x.unit_end = pos;
-    }
+    for (BWave& x : waves) list_units(bq, &x, b->host_units, &pos);
     BHeader hd{};
     hd.nq = nq;
     hd.nwaves = nw;
@@ -866,7 +923,7 @@ int qk_batch_create(qk_batch* out, const qk_batch_columns* cols) {
     qk_batch_s* b = new (std::nothrow) qk_batch_s();
     if (!b) return fail(QK_OUT_OF_MEMORY, "qk_batch_create: host allocation");
     b->cols = *cols;
-    b->tune.tile_rows = 1 << 20;
+    b->tune.tile_rows = 1 << 18;
     b->tune.lds_groups = 4096;
     b->tune.wgs_per_cu = 1;
     b->tune.use_graph = 1;
@@ -889,7 +946,7 @@ int qk_batch_set_tuning(qk_batch b, const qk_batch_tuning* t) {
     if (t->scratch_rows < 0) return fail(QK_ERROR, "qk_batch_set_tuning: scratch_rows must be >= 0");
     const bool same = memcmp(&b->tune, t, sizeof *t) == 0;
     b->tune = *t;
-    const int64_t rows = t->scratch_rows > 0 ? t->scratch_rows : (int64_t)1 << 30;
+    const int64_t rows = t->scratch_rows > 0 ? t->scratch_rows : kDefaultScratchRows;
     if (rows != b->scratch_rows) {
         if (b->sk) pool_free(b->sk);
         if (b->sv) pool_free(b->sv);
@@ -921,7 +978,7 @@ int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_
     hipStream_t s = as_stream(stream);
     const qk_batch_tuning keep = b->tune;
     std::vector<qk_batch_tuning> cands;
-    for (int tile : {1 << 18, 1 << 20, 1 << 22})
+    for (int tile : {1 << 16, 1 << 17, 1 << 18, 1 << 19})
         for (int groups : {2048, 4096, 5120})
             for (int wpc : {1, 2}) {
                 qk_batch_tuning t{tile, groups, wpc, keep.use_graph, keep.scratch_rows};

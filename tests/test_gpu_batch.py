@@ -60,7 +60,7 @@ def check_all(d, eng, qs):
 
 
 @pytest.mark.parametrize("tuning", [B.Tuning(1 << 18, 4096, 1, 1), B.Tuning(1 << 16, 2048, 2, 0),
-                                    B.Tuning(1 << 20, 5120, 1, 0), B.Tuning(4096, 0, 1, 1),
+                                    B.Tuning(1 << 19, 5120, 1, 0), B.Tuning(4096, 0, 1, 1),
                                     B.Tuning(1 << 16, 2048, 1, 1, scratch_rows=200_000),
                                     B.Tuning(1 << 16, 2048, 1, 0, scratch_rows=5_000)],
                          ids=["default", "small-tiles-2wg", "big-tiles", "all-two-pass", "waves",
