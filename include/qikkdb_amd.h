@@ -346,9 +346,14 @@ typedef struct qk_batch_tuning {
 
 typedef struct qk_batch_s* qk_batch;
 
-/* A batch engine over one table (columns stay owned by the caller). */
+/* A batch engine over one table (columns stay owned by the caller).  Creating it takes sorted
+ * samples of ts from every block (16384 per block), from which the passing rows of a query are
+ * estimated on the host; qk_batch_resample takes them again after the caller changes ts. */
 int qk_batch_create(qk_batch* out, const qk_batch_columns* cols);
 int qk_batch_destroy(qk_batch b);
+int qk_batch_resample(qk_batch b);
+/* Estimated passing rows of each query (host only, no device work). */
+int qk_batch_estimate_rows(qk_batch b, int32_t nq, const qk_batch_query* q, int64_t* out);
 int qk_batch_set_tuning(qk_batch b, const qk_batch_tuning* t);
 int qk_batch_get_tuning(qk_batch b, qk_batch_tuning* t);
 /* Time candidate tunings on the given queries (they are run several times) and keep the
@@ -371,6 +376,59 @@ int qk_batch_completion_ms(qk_batch b, double* out, int32_t nq);
  * time caller materialises before GROUP BY. */
 int qk_batch_key_column(const int32_t* a, const int32_t* b, int32_t b_range, int32_t key_mod,
                         int32_t n, int64_t* out, void* stream);
+
+/* ======================================================================================
+ * Serving (no reference counterpart: the reference gives each client connection a worker
+ * thread that runs its queries one at a time, ClientPoolWorker.cpp:42-140)
+ *
+ * Continuous batching of requests of the batched form above.  Requests are queued as they are
+ * submitted; a dispatcher thread runs the oldest ones as a batch on a free engine (one stream
+ * each) while the projected bytes of all admitted, unreleased requests stay under memory_cap,
+ * and starts the next batch while the previous one is still on the device.  Projected bytes
+ * of a request: key_mod x 32 B of dense result table, plus (estimated rows x 1.25 + 4096) x
+ * 12 B of two-pass pool when key_mod > tuning.lds_groups.  A request is done when its batch
+ * has finished on the device; its groups stay in device memory until it is released, and the
+ * engine of a batch takes new work once all the batch's requests are released.
+ * ====================================================================================== */
+
+typedef struct qk_serve_config {
+    int32_t engines;          /* batch engines: batches running or holding results (0 = 2) */
+    int32_t max_batch;        /* most requests per batch (0 = 4096) */
+    int64_t memory_cap;       /* projected bytes admitted at once (0 = 200 GB) */
+    double wait_ms;           /* with a batch running, the next one starts when its oldest
+                                 request has waited this long (0 = half the last batch's time) */
+    qk_batch_tuning tuning;   /* of every engine (tile_rows 0 = defaults, no graph replay) */
+} qk_serve_config;
+
+typedef struct qk_serve_stats {
+    int64_t batches, served, failed, pending;
+    int64_t max_batch;                          /* largest batch run */
+    int64_t admitted_bytes, peak_admitted_bytes;
+    double busy_ms;                             /* sum of batch times */
+} qk_serve_stats;
+
+typedef struct qk_serve_s* qk_serve;
+
+int qk_serve_create(qk_serve* out, const qk_batch_columns* cols, const qk_serve_config* cfg);
+/* Waits for the batches on the device; requests not yet started fail. */
+int qk_serve_destroy(qk_serve s);
+/* Thread-safe.  QK_OUT_OF_MEMORY when the request alone projects more than memory_cap. */
+int qk_serve_submit(qk_serve s, const qk_batch_query* q, int64_t* ticket);
+/* Waits until the request is done (timeout_ms < 0: no limit); returns its error if it failed. */
+int qk_serve_wait(qk_serve s, int64_t ticket, double timeout_ms);
+/* Device pointers to groups [0, key_mod) of a done request (as qk_batch_results), valid
+ * until qk_serve_release. */
+int qk_serve_result(qk_serve s, int64_t ticket, const int64_t** count, const int64_t** sum,
+                    const int64_t** min, const int64_t** max);
+/* Submit-to-done time of a done request, in ms. */
+int qk_serve_latency_ms(qk_serve s, int64_t ticket, double* out);
+/* Drops a done or pending request (a running one cannot be released). */
+int qk_serve_release(qk_serve s, int64_t ticket);
+/* Open-loop replay: submits request i at arrival_ms[i] (ms from the call, non-decreasing),
+ * waits for all and stores each one's latency from its scheduled arrival to done. */
+int qk_serve_replay(qk_serve s, int32_t n, const qk_batch_query* q, const double* arrival_ms,
+                    double* latency_ms);
+int qk_serve_get_stats(qk_serve s, qk_serve_stats* out);
 
 #ifdef __cplusplus
 }

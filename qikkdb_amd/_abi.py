@@ -106,6 +106,33 @@ class BatchTuning(C.Structure):
     ]
 
 
+class ServeConfig(C.Structure):
+    """qk_serve_config."""
+
+    _fields_ = [
+        ("engines", C.c_int32),
+        ("max_batch", C.c_int32),
+        ("memory_cap", C.c_int64),
+        ("wait_ms", C.c_double),
+        ("tuning", BatchTuning),
+    ]
+
+
+class ServeStats(C.Structure):
+    """qk_serve_stats."""
+
+    _fields_ = [
+        ("batches", C.c_int64),
+        ("served", C.c_int64),
+        ("failed", C.c_int64),
+        ("pending", C.c_int64),
+        ("max_batch", C.c_int64),
+        ("admitted_bytes", C.c_int64),
+        ("peak_admitted_bytes", C.c_int64),
+        ("busy_ms", C.c_double),
+    ]
+
+
 _vp, _i32, _i64, _u32, _sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_size_t
 _P = C.POINTER
 
@@ -161,6 +188,17 @@ PROTOTYPES = {
     "qk_batch_results": [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp)],
     "qk_batch_completion_ms": [_vp, _P(C.c_double), _i32],
     "qk_batch_key_column": [_vp, _vp, _i32, _i32, _i32, _vp, _vp],
+    "qk_batch_resample": [_vp],
+    "qk_batch_estimate_rows": [_vp, _i32, _P(BatchQuery), _P(_i64)],
+    "qk_serve_create": [_P(_vp), _P(BatchColumns), _P(ServeConfig)],
+    "qk_serve_destroy": [_vp],
+    "qk_serve_submit": [_vp, _P(BatchQuery), _P(_i64)],
+    "qk_serve_wait": [_vp, _i64, C.c_double],
+    "qk_serve_result": [_vp, _i64, _P(_vp), _P(_vp), _P(_vp), _P(_vp)],
+    "qk_serve_latency_ms": [_vp, _i64, _P(C.c_double)],
+    "qk_serve_release": [_vp, _i64],
+    "qk_serve_replay": [_vp, _i32, _P(BatchQuery), _P(C.c_double), _P(C.c_double)],
+    "qk_serve_get_stats": [_vp, _P(ServeStats)],
 }
 
 _lib = None

@@ -92,6 +92,13 @@ class BatchEngine:
         call("qk_batch_run", self.h, n, arr, stream)
         self.last = n
 
+    def estimate_rows(self, qs: Sequence[Query]) -> np.ndarray:
+        """Estimated passing rows of each query, from the ts samples taken at creation."""
+        arr, n = _queries(qs)
+        out = np.zeros(max(1, n), np.int64)
+        call("qk_batch_estimate_rows", self.h, n, arr, out.ctypes.data_as(C.POINTER(C.c_int64)))
+        return out[:n]
+
     def completion_ms(self) -> np.ndarray:
         out = np.zeros(self.last, np.float64)
         call("qk_batch_completion_ms", self.h, out.ctypes.data_as(C.POINTER(C.c_double)), self.last)

@@ -36,7 +36,7 @@ namespace {
 
 constexpr int kBThreads = 1024;
 constexpr int kBRows = 4;            // rows per thread per step of a unit
-constexpr int kSamples = 4096;       // rows sampled per large query for its estimate
+constexpr int kSamples = 16384;      // ts samples per block for the row estimates
 constexpr int kMaxKeyBits = 24;      // key_mod <= 2^24
 constexpr int kMinChunkBits = 6;     // narrowest key chunk of a large query: 64 groups
 constexpr int64_t kDefaultScratchRows = (int64_t)1 << 32;  // 48 GiB at most, allocated as needed
@@ -101,7 +101,6 @@ struct BArgs {
     int64_t* segb;
     uint32_t* sk;               // scratch pool: keys
     int64_t* sv;                // scratch pool: values
-    int64_t* est;               // [nq] estimated passing rows (sampling)
     unsigned long long* cnt;
     unsigned long long* sum;
     unsigned long long* mn;
@@ -188,24 +187,18 @@ __global__ void __launch_bounds__(256) k_batch_finalize(BArgs p) {
     }
 }
 
-// One workgroup per query: passing rows among kSamples rows spread over its range, scaled.
-__global__ void __launch_bounds__(256) k_batch_estimate(BArgs p, int32_t nq) {
-    __shared__ uint32_t hits;
-    const int32_t i = blockIdx.x;
-    if (i >= nq) return;
-    const BQuery Q = p.q[i];
-    if (threadIdx.x == 0) hits = 0;
-    __syncthreads();
-    const int64_t stride = max<int64_t>(1, Q.rows / kSamples);
-    uint32_t mine = 0;
-    for (int s = threadIdx.x; s < kSamples; s += blockDim.x) {
-        const int64_t r = Q.row0 + min<int64_t>((int64_t)s * stride, Q.rows - 1);
-        const int64_t t = p.ts[r];
-        mine += t >= Q.ts_lo && t < Q.ts_hi;
+// ts samples of every block: out[blk * per + i] = ts[row of sample i of block blk], the
+// samples spread evenly over the block.
+__global__ void __launch_bounds__(256) k_batch_sample_ts(const int64_t* __restrict__ ts, int64_t rows,
+                                                         int64_t block_rows, int32_t per, int64_t nblocks,
+                                                         int64_t* __restrict__ out) {
+    const int64_t n = nblocks * per;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t blk = i / per, j = i % per;
+        const int64_t r0 = blk * block_rows, len = min(block_rows, rows - r0);
+        out[i] = ts[r0 + j * len / per];
     }
-    atomicAdd(&hits, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) p.est[i] = (int64_t)(((double)hits / kSamples) * (double)Q.rows);
 }
 
 template <typename F>
@@ -499,7 +492,6 @@ struct qk_batch_s {
     void* plan = nullptr;  // BHeader, BWave[wave_cap], BQuery[q_cap]
     uint32_t* done = nullptr;
     unsigned long long* comp = nullptr;
-    int64_t* est = nullptr;
     int32_t q_cap = 0;
     unsigned long long* ctr = nullptr;  // [0] start clock, then 3 per wave
     int32_t wave_cap = 0;
@@ -520,6 +512,9 @@ struct qk_batch_s {
     int64_t host_unit_cap = 0;
     std::vector<int64_t> offsets;
     std::vector<int64_t> est_host;
+    // sorted ts samples of each block (taken at creation) for the row estimates
+    std::vector<int64_t> samples;
+    int32_t per_block = 0;
     int32_t last_nq = 0;
     // captured graph of init -> waves -> finalize, valid while the launch arguments match
     hipGraphExec_t graph = nullptr;
@@ -553,13 +548,12 @@ int grow(T** p, int64_t* cap, int64_t need, int64_t min_cap) {
 }
 
 void free_plan(qk_batch_s* b) {
-    void* ptrs[] = {b->plan, b->done, b->comp, b->est, b->ctr};
+    void* ptrs[] = {b->plan, b->done, b->comp, b->ctr};
     for (void* p : ptrs)
         if (p) pool_free(p);
     b->plan = nullptr;
     b->done = nullptr;
     b->comp = nullptr;
-    b->est = nullptr;
     b->ctr = nullptr;
     if (b->host_plan) (void)hipHostFree(b->host_plan);
     b->host_plan = nullptr;
@@ -606,7 +600,6 @@ BArgs args_of(const qk_batch_s* b) {
     p.segb = b->segb;
     p.sk = b->sk;
     p.sv = b->sv;
-    p.est = b->est;
     p.cnt = b->cnt;
     p.sum = b->sum;
     p.mn = b->mn;
@@ -678,7 +671,6 @@ int ensure_plan(qk_batch_s* b, int32_t nq, int32_t nwaves) {
     QK_TRY(pool_alloc(&b->plan, pc));
     QK_TRY(pool_alloc((void**)&b->done, (size_t)qc * 4));
     QK_TRY(pool_alloc((void**)&b->comp, (size_t)qc * 8));
-    QK_TRY(pool_alloc((void**)&b->est, (size_t)qc * 8));
     QK_TRY(pool_alloc((void**)&b->ctr, (size_t)(1 + kCtrStride * wc) * 8));
     if (hipHostMalloc(&b->host_plan, pc) != hipSuccess) {
         b->host_plan = nullptr;
@@ -742,6 +734,43 @@ void list_units(const std::vector<BQuery>& bq, BWave* w, BUnit* out, int64_t* n)
     *n = pos;
 }
 
+// Sorted ts samples of every block, taken once (the table is device-resident and read-only
+// for the engine; qk_batch_resample takes them again after the caller changes it).
+int take_samples(qk_batch_s* b) {
+    const qk_batch_columns& c = b->cols;
+    const int64_t nblocks = (c.rows + c.block_rows - 1) / c.block_rows;
+    const int32_t per = (int32_t)std::min<int64_t>(kSamples, c.block_rows);
+    b->samples.assign((size_t)(nblocks * per), 0);
+    b->per_block = per;
+    if (nblocks == 0) return QK_SUCCESS;
+    int64_t* dev = nullptr;
+    QK_TRY(pool_alloc((void**)&dev, (size_t)(nblocks * per) * 8));
+    const int grid = (int)std::min<int64_t>((nblocks * per + 255) / 256, 4096);
+    k_batch_sample_ts<<<grid, 256>>>(c.ts, c.rows, c.block_rows, per, nblocks, dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(b->samples.data(), dev, (size_t)(nblocks * per) * 8, hipMemcpyDeviceToHost);
+    pool_free(dev);
+    if (e != hipSuccess) return fail(QK_RUNTIME_ERROR, "qk_batch: sampling ts: %s", hipGetErrorString(e));
+    for (int64_t blk = 0; blk < nblocks; blk++)
+        std::sort(b->samples.begin() + blk * per, b->samples.begin() + (blk + 1) * per);
+    return QK_SUCCESS;
+}
+
+// Passing rows of a query estimated from the samples of its blocks.
+int64_t estimate_rows(const qk_batch_s* b, const qk_batch_query& x) {
+    const qk_batch_columns& c = b->cols;
+    const int32_t per = b->per_block;
+    if (per == 0 || x.ts_hi <= x.ts_lo) return 0;
+    double rows = 0.0;
+    for (int64_t blk = x.first_block; blk < (int64_t)x.first_block + x.nblocks; blk++) {
+        const int64_t* s0 = b->samples.data() + blk * per;
+        const int64_t hit = std::lower_bound(s0, s0 + per, x.ts_hi) - std::lower_bound(s0, s0 + per, x.ts_lo);
+        const int64_t len = std::min<int64_t>(c.block_rows, c.rows - blk * c.block_rows);
+        rows += (double)hit * (double)len / per;
+    }
+    return (int64_t)rows;
+}
+
 int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s) {
     const qk_batch_columns& c = b->cols;
     const qk_batch_tuning& t = b->tune;
@@ -750,6 +779,8 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
     const int cbmax = chunk_bits_of(t.lds_groups);
     if (nq > (1 << 24)) return fail(QK_ERROR, "qk_batch: at most 2^24 queries per run");
     if ((c.rows + T - 1) / T > INT32_MAX) return fail(QK_ERROR, "qk_batch: too many tiles; raise tile_rows");
+    // the previous run of this engine may still read the buffers and pinned plan changed below
+    QK_HIP(hipStreamSynchronize(s));
     // queries in plan order: small ones first, then large ones (waves are cut below)
     std::vector<int32_t> order;
     order.reserve((size_t)nq);
@@ -802,22 +833,9 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         b->out_cap = cap;
     }
     QK_TRY(ensure_plan(b, nq, std::max(1, nlarge)));
-    QK_HIP(hipStreamSynchronize(s));  // the pinned plan and unit list are rewritten below
-    BQuery* host_q = (BQuery*)((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
-    const BArgs p0 = args_of(b);
-    // estimate the passing rows of the large queries (sampling) to size chunks and cut waves
-    b->est_host.assign((size_t)nlarge, 0);
-    if (nlarge) {
-        memcpy(host_q + nsmall, bq.data() + nsmall, sizeof(BQuery) * (size_t)nlarge);
-        QK_HIP(hipMemcpyAsync((void*)(p0.q + nsmall), host_q + nsmall, sizeof(BQuery) * (size_t)nlarge,
-                              hipMemcpyHostToDevice, s));
-        BArgs ep = p0;
-        ep.q += nsmall;
-        k_batch_estimate<<<nlarge, 256, 0, s>>>(ep, nlarge);
-        QK_HIP(hipGetLastError());
-        QK_HIP(hipMemcpyAsync(b->est_host.data(), b->est, (size_t)nlarge * 8, hipMemcpyDeviceToHost, s));
-        QK_HIP(hipStreamSynchronize(s));
-    }
+    // estimated passing rows of the large queries, to size the pool and chunks and cut waves
+    b->est_host.resize((size_t)nlarge);
+    for (int32_t k = nsmall; k < nq; k++) b->est_host[(size_t)(k - nsmall)] = estimate_rows(b, q[bq[(size_t)k].index]);
     // the pool: the batch's estimated rows up to scratch_rows (grown, never shrunk)
     int64_t need_all = 0;
     for (int64_t e : b->est_host) need_all += e + e / 4 + 4096;
@@ -886,7 +904,7 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
     hd.out_total = out;
     hd.scratch_cap = nlarge ? pool : 0;
     hd.nunits = nunits;
-    host_q = (BQuery*)((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
+    BQuery* host_q = (BQuery*)((char*)b->host_plan + sizeof(BHeader) + sizeof(BWave) * (size_t)b->wave_cap);
     memcpy(b->host_plan, &hd, sizeof hd);
     memcpy((char*)b->host_plan + sizeof hd, waves.data(), sizeof(BWave) * (size_t)nw);
     if (nq) memcpy(host_q, bq.data(), sizeof(BQuery) * (size_t)nq);
@@ -928,7 +946,29 @@ int qk_batch_create(qk_batch* out, const qk_batch_columns* cols) {
     b->tune.wgs_per_cu = 1;
     b->tune.use_graph = 1;
     b->tune.scratch_rows = 0;
+    const int st = take_samples(b);
+    if (st != QK_SUCCESS) {
+        delete b;
+        return st;
+    }
     *out = b;
+    return QK_SUCCESS;
+}
+
+int qk_batch_resample(qk_batch b) {
+    if (!b) return fail(QK_ERROR, "qk_batch_resample: null batch");
+    QK_TRY(ensure_init());
+    return take_samples(b);
+}
+
+int qk_batch_estimate_rows(qk_batch b, int32_t nq, const qk_batch_query* q, int64_t* out) {
+    if (!b || nq < 0 || (nq > 0 && (!q || !out))) return fail(QK_ERROR, "qk_batch_estimate_rows: bad argument");
+    const int64_t nblocks = (b->cols.rows + b->cols.block_rows - 1) / b->cols.block_rows;
+    for (int32_t i = 0; i < nq; i++) {
+        if (q[i].first_block < 0 || q[i].nblocks < 1 || q[i].first_block + (int64_t)q[i].nblocks > nblocks)
+            return fail(QK_ERROR, "qk_batch_estimate_rows: query %d: block range outside the table", i);
+        out[i] = estimate_rows(b, q[i]);
+    }
     return QK_SUCCESS;
 }
 

@@ -1,0 +1,459 @@
+// serve.hip - continuous batching of filtered GROUP BY requests over one device-resident table
+// (qk_serve_* in include/qikkdb_amd.h).  Host code only: the device work is the batch engine's.
+//
+// Requests are queued as they are submitted.  A dispatcher thread owns `engines` batch
+// engines, each with its own stream.  Whenever an engine is free it takes the oldest requests
+// (first come, first served) while the projected bytes of everything admitted and not yet
+// released stay under the memory cap, and runs them as one batch.  A new batch is started
+// while another is still on the device, so the two overlap there.  It is started when the
+// device is idle, when a full batch is waiting, or when the oldest request has waited
+// `wait_ms`.  A request is done when its batch's completion event has fired; its groups stay in
+// the engine's result arrays until the caller releases it, and an engine takes a new batch once
+// every request of its last batch is released.
+//
+// Projected bytes of a request: its dense result table, key_mod groups x 32 B (count, sum, min,
+// max; a dense table has load factor 1), plus, for a request with more than lds_groups groups,
+// its share of the two-pass pool: (estimated rows x 1.25 + 4096) x 12 B.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "common.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum class TState { Pending, Running, Done, Failed };
+enum class EState { Free, Running, Holding };
+
+struct Ticket {
+    qk_batch_query q{};
+    int64_t bytes = 0;      // projected
+    double t_submit = 0.0;  // ms since the server started
+    double t_due = 0.0;     // latency is measured from here (the submit time, or a scheduled arrival)
+    double t_done = 0.0;
+    TState state = TState::Pending;
+    int err = QK_SUCCESS;
+    std::string msg;
+    int engine = -1;
+    int32_t slot = -1;      // position in its batch
+    bool auto_release = false;
+    double* latency_out = nullptr;  // auto-release: where the latency goes
+};
+
+struct Engine {
+    qk_batch b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t done = nullptr;
+    EState state = EState::Free;
+    std::vector<int64_t> tickets;
+    int32_t live = 0;       // tickets of the batch not yet released
+    int64_t bytes = 0;      // projected bytes of the batch
+    double t_start = 0.0;
+};
+
+}  // namespace
+
+struct qk_serve_s {
+    qk_batch_columns cols{};
+    qk_serve_config cfg{};
+    int device = 0;
+    Clock::time_point t0;
+    std::vector<Engine> eng;
+    std::mutex mu;
+    std::condition_variable cv_work;  // the dispatcher waits here
+    std::condition_variable cv_done;  // callers of qk_serve_wait wait here
+    std::deque<int64_t> pending;
+    std::unordered_map<int64_t, Ticket> tickets;
+    int64_t next_ticket = 1;
+    int64_t admitted_bytes = 0;       // projected bytes of dispatched, unreleased batches
+    int64_t outstanding_auto = 0;     // auto-release tickets not yet done
+    bool stop = false;
+    std::thread th;
+    // statistics
+    int64_t batches = 0, served = 0, failed = 0, max_batch_seen = 0, peak_bytes = 0;
+    double busy_ms = 0.0, last_batch_ms = 0.0;
+};
+
+namespace {
+
+double now_ms(const qk_serve_s* s) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - s->t0).count();
+}
+
+// Finishes ticket `id` of a completed batch (lock held).
+void finish(qk_serve_s* s, Engine& e, int64_t id, double t, int err, const std::string& msg) {
+    Ticket& k = s->tickets[id];
+    k.t_done = t;
+    k.state = err == QK_SUCCESS ? TState::Done : TState::Failed;
+    k.err = err;
+    k.msg = msg;
+    if (err == QK_SUCCESS) s->served++;
+    else s->failed++;
+    if (k.auto_release) {
+        if (k.latency_out) *k.latency_out = err == QK_SUCCESS ? t - k.t_due : -1.0;
+        s->tickets.erase(id);
+        e.live--;
+        s->outstanding_auto--;
+    }
+}
+
+// Fails every request not yet dispatched (lock held): the server is stopping.
+void drop_pending(qk_serve_s* s) {
+    const double t = now_ms(s);
+    for (int64_t id : s->pending) {
+        Ticket& k = s->tickets[id];
+        k.t_done = t;
+        k.state = TState::Failed;
+        k.err = QK_ERROR;
+        k.msg = "qk_serve: the server stopped before the request ran";
+        s->failed++;
+        if (k.auto_release) {
+            if (k.latency_out) *k.latency_out = -1.0;
+            s->tickets.erase(id);
+            s->outstanding_auto--;
+        }
+    }
+    s->pending.clear();
+}
+
+void release_engine_if_idle(qk_serve_s* s, Engine& e) {
+    if (e.state == EState::Holding && e.live == 0) {
+        e.state = EState::Free;
+        s->admitted_bytes -= e.bytes;
+        e.bytes = 0;
+        e.tickets.clear();
+    }
+}
+
+// Marks the batches whose completion event fired (lock held).  Returns true if any did.
+bool collect(qk_serve_s* s) {
+    bool any = false;
+    for (Engine& e : s->eng) {
+        if (e.state != EState::Running) continue;
+        const hipError_t q = hipEventQuery(e.done);
+        if (q == hipErrorNotReady) continue;
+        const double t = now_ms(s);
+        int err = QK_SUCCESS;
+        std::string msg;
+        if (q != hipSuccess) {
+            err = QK_RUNTIME_ERROR;
+            msg = std::string("qk_serve: batch failed on the device: ") + hipGetErrorString(q);
+            (void)hipGetLastError();
+        }
+        e.state = EState::Holding;
+        s->batches++;
+        s->last_batch_ms = t - e.t_start;
+        s->busy_ms += s->last_batch_ms;
+        for (int64_t id : e.tickets) finish(s, e, id, t, err, msg);
+        release_engine_if_idle(s, e);
+        any = true;
+    }
+    return any;
+}
+
+void dispatcher(qk_serve_s* s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    std::vector<qk_batch_query> qs;
+    while (true) {
+        if (collect(s)) s->cv_done.notify_all();
+        int running = 0;
+        Engine* free_e = nullptr;
+        for (Engine& e : s->eng) {
+            running += e.state == EState::Running;
+            if (e.state == EState::Free && !free_e) free_e = &e;
+        }
+        if (s->stop) {
+            if (!s->pending.empty()) {
+                drop_pending(s);
+                s->cv_done.notify_all();
+            }
+            if (running == 0) break;
+        }
+        bool launched = false;
+        if (free_e && !s->pending.empty()) {
+            const double t = now_ms(s);
+            const double waited = t - s->tickets[s->pending.front()].t_submit;
+            const double wait_ms = s->cfg.wait_ms > 0 ? s->cfg.wait_ms : 0.5 * s->last_batch_ms;
+            if (running == 0 || (int64_t)s->pending.size() >= s->cfg.max_batch || waited >= wait_ms) {
+                // admit the oldest requests while their projected bytes fit
+                Engine& e = *free_e;
+                int64_t bytes = 0;
+                e.tickets.clear();
+                qs.clear();
+                while (!s->pending.empty() && (int64_t)qs.size() < s->cfg.max_batch) {
+                    Ticket& k = s->tickets[s->pending.front()];
+                    if (s->admitted_bytes + bytes + k.bytes > s->cfg.memory_cap) break;
+                    bytes += k.bytes;
+                    k.state = TState::Running;
+                    k.engine = (int)(&e - s->eng.data());
+                    k.slot = (int32_t)qs.size();
+                    qs.push_back(k.q);
+                    e.tickets.push_back(s->pending.front());
+                    s->pending.pop_front();
+                }
+                if (!qs.empty()) {
+                    e.state = EState::Running;
+                    e.live = (int32_t)qs.size();
+                    e.bytes = bytes;
+                    e.t_start = t;
+                    s->admitted_bytes += bytes;
+                    s->peak_bytes = std::max(s->peak_bytes, s->admitted_bytes);
+                    s->max_batch_seen = std::max<int64_t>(s->max_batch_seen, (int64_t)qs.size());
+                    lk.unlock();
+                    int st = qk_batch_run(e.b, (int32_t)qs.size(), qs.data(), e.s);
+                    std::string msg;
+                    if (st == QK_SUCCESS && hipEventRecord(e.done, e.s) != hipSuccess) {
+                        (void)hipGetLastError();
+                        st = QK_RUNTIME_ERROR;
+                        msg = "qk_serve: hipEventRecord failed";
+                    } else if (st != QK_SUCCESS) {
+                        msg = qk_last_error_message();
+                    }
+                    lk.lock();
+                    if (st != QK_SUCCESS) {
+                        const double tf = now_ms(s);
+                        e.state = EState::Holding;
+                        for (int64_t id : e.tickets) finish(s, e, id, tf, st, msg);
+                        release_engine_if_idle(s, e);
+                        s->cv_done.notify_all();
+                    }
+                    launched = true;
+                }
+            }
+        }
+        if (launched) continue;
+        // completion events are polled; arrivals, releases and stop wake the wait early
+        if (running > 0 || !s->pending.empty()) s->cv_work.wait_for(lk, std::chrono::microseconds(100));
+        else s->cv_work.wait(lk);
+    }
+}
+
+int64_t projected_bytes(const qk_serve_s* s, const qk_batch_query& q, int64_t est_rows) {
+    int64_t b = (int64_t)q.key_mod * 32;
+    if (q.key_mod > s->cfg.tuning.lds_groups) b += (est_rows + est_rows / 4 + 4096) * 12;
+    return b;
+}
+
+int submit(qk_serve_s* s, const qk_batch_query* q, double due_ms, bool auto_release, double* latency_out,
+           int64_t* ticket) {
+    int64_t est = 0;
+    QK_TRY(qk_batch_estimate_rows(s->eng[0].b, 1, q, &est));  // validates the block range too
+    if (q->key_mod < 1 || q->key_mod > (1 << 24)) return qk::fail(QK_ERROR, "qk_serve_submit: key_mod must be 1..2^24");
+    const int64_t bytes = projected_bytes(s, *q, est);
+    if (bytes > s->cfg.memory_cap)
+        return qk::fail(QK_OUT_OF_MEMORY, "qk_serve_submit: projected %lld bytes exceed the cap of %lld",
+                        (long long)bytes, (long long)s->cfg.memory_cap);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return qk::fail(QK_ERROR, "qk_serve_submit: server is stopping");
+        const int64_t id = s->next_ticket++;
+        Ticket& k = s->tickets[id];
+        k.q = *q;
+        k.bytes = bytes;
+        k.t_submit = now_ms(s);
+        k.t_due = due_ms >= 0 ? due_ms : k.t_submit;
+        k.auto_release = auto_release;
+        k.latency_out = latency_out;
+        s->pending.push_back(id);
+        if (auto_release) s->outstanding_auto++;
+        if (ticket) *ticket = id;
+    }
+    s->cv_work.notify_one();
+    return QK_SUCCESS;
+}
+
+void destroy(qk_serve_s* s) {
+    if (s->th.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv_work.notify_all();
+        s->th.join();
+    }
+    for (Engine& e : s->eng) {
+        if (e.s) (void)hipStreamSynchronize(e.s);
+        if (e.b) (void)qk_batch_destroy(e.b);
+        if (e.done) (void)hipEventDestroy(e.done);
+        if (e.s) (void)hipStreamDestroy(e.s);
+    }
+    delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qk_serve_create(qk_serve* out, const qk_batch_columns* cols, const qk_serve_config* cfg) {
+    if (!out || !cols) return qk::fail(QK_ERROR, "qk_serve_create: null argument");
+    *out = nullptr;
+    QK_TRY(qk::ensure_init());
+    qk_serve_config c{};
+    if (cfg) c = *cfg;
+    if (c.engines == 0) c.engines = 2;
+    if (c.max_batch == 0) c.max_batch = 4096;
+    if (c.memory_cap == 0) c.memory_cap = (int64_t)200 * 1000 * 1000 * 1000;
+    if (c.tuning.tile_rows == 0) {
+        c.tuning.tile_rows = 1 << 18;
+        c.tuning.lds_groups = 4096;
+        c.tuning.wgs_per_cu = 1;
+        c.tuning.use_graph = 0;
+        c.tuning.scratch_rows = 0;
+    }
+    if (c.engines < 1 || c.engines > 8 || c.max_batch < 1 || c.memory_cap < 1 || c.wait_ms < 0)
+        return qk::fail(QK_ERROR, "qk_serve_create: engines 1..8, max_batch >= 1, memory_cap >= 1, wait_ms >= 0");
+    qk_serve_s* s = new (std::nothrow) qk_serve_s();
+    if (!s) return qk::fail(QK_OUT_OF_MEMORY, "qk_serve_create: host allocation");
+    s->cols = *cols;
+    s->cfg = c;
+    s->t0 = Clock::now();
+    (void)hipGetDevice(&s->device);
+    s->eng.resize((size_t)c.engines);
+    int st = QK_SUCCESS;
+    for (Engine& e : s->eng) {
+        st = qk_batch_create(&e.b, cols);
+        if (st == QK_SUCCESS) st = qk_batch_set_tuning(e.b, &c.tuning);
+        if (st == QK_SUCCESS && (hipStreamCreateWithFlags(&e.s, hipStreamNonBlocking) != hipSuccess ||
+                                 hipEventCreateWithFlags(&e.done, hipEventDisableTiming) != hipSuccess)) {
+            (void)hipGetLastError();
+            st = qk::fail(QK_RUNTIME_ERROR, "qk_serve_create: stream or event creation failed");
+        }
+        if (st != QK_SUCCESS) break;
+    }
+    if (st != QK_SUCCESS) {
+        destroy(s);
+        return st;
+    }
+    s->th = std::thread(dispatcher, s);
+    *out = s;
+    return QK_SUCCESS;
+}
+
+int qk_serve_destroy(qk_serve s) {
+    if (!s) return QK_SUCCESS;
+    destroy(s);
+    return QK_SUCCESS;
+}
+
+int qk_serve_submit(qk_serve s, const qk_batch_query* q, int64_t* ticket) {
+    if (!s || !q || !ticket) return qk::fail(QK_ERROR, "qk_serve_submit: null argument");
+    return submit(s, q, -1.0, false, nullptr, ticket);
+}
+
+int qk_serve_wait(qk_serve s, int64_t ticket, double timeout_ms) {
+    if (!s) return qk::fail(QK_ERROR, "qk_serve_wait: null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end() || it->second.auto_release) return qk::fail(QK_ERROR, "qk_serve_wait: unknown ticket");
+    const auto ready = [&] {
+        auto i = s->tickets.find(ticket);
+        return i == s->tickets.end() || i->second.state == TState::Done || i->second.state == TState::Failed;
+    };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+        return qk::fail(QK_ERROR, "qk_serve_wait: timed out");
+    it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return qk::fail(QK_ERROR, "qk_serve_wait: ticket released while waiting");
+    if (it->second.state == TState::Failed) return qk::fail(it->second.err, "%s", it->second.msg.c_str());
+    return QK_SUCCESS;
+}
+
+int qk_serve_result(qk_serve s, int64_t ticket, const int64_t** count, const int64_t** sum, const int64_t** min,
+                    const int64_t** max) {
+    if (!s) return qk::fail(QK_ERROR, "qk_serve_result: null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end() || it->second.state != TState::Done)
+        return qk::fail(QK_ERROR, "qk_serve_result: ticket %lld is not done", (long long)ticket);
+    const Ticket& k = it->second;
+    const int64_t *c = nullptr, *sm = nullptr, *mn = nullptr, *mx = nullptr, *off = nullptr;
+    QK_TRY(qk_batch_results(s->eng[(size_t)k.engine].b, &c, &sm, &mn, &mx, &off));
+    const int64_t o = off[k.slot];
+    if (count) *count = c + o;
+    if (sum) *sum = sm + o;
+    if (min) *min = mn + o;
+    if (max) *max = mx + o;
+    return QK_SUCCESS;
+}
+
+int qk_serve_latency_ms(qk_serve s, int64_t ticket, double* out) {
+    if (!s || !out) return qk::fail(QK_ERROR, "qk_serve_latency_ms: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end() || it->second.state != TState::Done)
+        return qk::fail(QK_ERROR, "qk_serve_latency_ms: ticket %lld is not done", (long long)ticket);
+    *out = it->second.t_done - it->second.t_due;
+    return QK_SUCCESS;
+}
+
+int qk_serve_release(qk_serve s, int64_t ticket) {
+    if (!s) return qk::fail(QK_ERROR, "qk_serve_release: null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->tickets.find(ticket);
+        if (it == s->tickets.end() || it->second.auto_release)
+            return qk::fail(QK_ERROR, "qk_serve_release: unknown ticket");
+        Ticket& k = it->second;
+        if (k.state == TState::Running) return qk::fail(QK_ERROR, "qk_serve_release: ticket is running");
+        if (k.state == TState::Pending) {
+            s->pending.erase(std::find(s->pending.begin(), s->pending.end(), ticket));
+        } else {
+            Engine& e = s->eng[(size_t)k.engine];
+            e.live--;
+            release_engine_if_idle(s, e);
+        }
+        s->tickets.erase(it);
+    }
+    s->cv_work.notify_one();
+    return QK_SUCCESS;
+}
+
+int qk_serve_replay(qk_serve s, int32_t n, const qk_batch_query* q, const double* arrival_ms, double* latency_ms) {
+    if (!s || n < 0 || (n > 0 && (!q || !arrival_ms || !latency_ms)))
+        return qk::fail(QK_ERROR, "qk_serve_replay: bad argument");
+    for (int32_t i = 1; i < n; i++)
+        if (arrival_ms[i] < arrival_ms[i - 1]) return qk::fail(QK_ERROR, "qk_serve_replay: arrivals must not decrease");
+    for (int32_t i = 0; i < n; i++) latency_ms[i] = -1.0;
+    const double start = now_ms(s);
+    int st = QK_SUCCESS;
+    for (int32_t i = 0; i < n && st == QK_SUCCESS; i++) {
+        const double due = start + arrival_ms[i];
+        const double wait = due - now_ms(s);
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
+        int64_t id = 0;
+        st = submit(s, q + i, due, true, latency_ms + i, &id);
+    }
+    // the submitted requests write latency_ms when they finish: wait for all of them
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->outstanding_auto == 0; });
+    if (st != QK_SUCCESS) return st;
+    for (int32_t i = 0; i < n; i++)
+        if (latency_ms[i] < 0) return qk::fail(QK_RUNTIME_ERROR, "qk_serve_replay: request %d failed", i);
+    return QK_SUCCESS;
+}
+
+int qk_serve_get_stats(qk_serve s, qk_serve_stats* out) {
+    if (!s || !out) return qk::fail(QK_ERROR, "qk_serve_get_stats: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->batches = s->batches;
+    out->served = s->served;
+    out->failed = s->failed;
+    out->pending = (int64_t)s->pending.size();
+    out->max_batch = s->max_batch_seen;
+    out->admitted_bytes = s->admitted_bytes;
+    out->peak_admitted_bytes = s->peak_bytes;
+    out->busy_ms = s->busy_ms;
+    return QK_SUCCESS;
+}
+
+}  // extern "C"

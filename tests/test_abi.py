@@ -82,3 +82,32 @@ def test_cpp_dropin_headers_compile(tmp_path):
         r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, src],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_c(tmp_path):
+    """The ctypes mirrors of the header's structs have the C sizes and field offsets."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {
+        "qk_batch_columns": _abi.BatchColumns, "qk_batch_query": _abi.BatchQuery,
+        "qk_batch_tuning": _abi.BatchTuning, "qk_serve_config": _abi.ServeConfig,
+        "qk_serve_stats": _abi.ServeStats, "qk_predicate": _abi.Predicate,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "qikkdb_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
