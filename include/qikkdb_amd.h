@@ -365,6 +365,11 @@ int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_
  * when its count is non-zero; MIN/MAX of an empty group are undefined).  Returns after
  * the work is queued; qk_stream_synchronize(stream) waits for it. */
 int qk_batch_run(qk_batch b, int32_t nq, const qk_batch_query* q, void* stream);
+/* As qk_batch_run, with the results written to the caller's device arrays (each at least
+ * sum(key_mod) entries, query i at the i-th prefix sum of key_mod) instead of the engine's;
+ * qk_batch_results then reports those arrays. */
+int qk_batch_run_into(qk_batch b, int32_t nq, const qk_batch_query* q, int64_t* count, int64_t* sum,
+                      int64_t* min, int64_t* max, void* stream);
 /* Device arrays of the last run and the host array of offsets (nq + 1 entries). */
 int qk_batch_results(qk_batch b, const int64_t** count, const int64_t** sum, const int64_t** min,
                      const int64_t** max, const int64_t** offsets_host);

@@ -185,6 +185,7 @@ PROTOTYPES = {
     "qk_batch_get_tuning": [_vp, _P(BatchTuning)],
     "qk_batch_autotune": [_vp, _i32, _P(BatchQuery), _P(BatchTuning), _vp],
     "qk_batch_run": [_vp, _i32, _P(BatchQuery), _vp],
+    "qk_batch_run_into": [_vp, _i32, _P(BatchQuery), _vp, _vp, _vp, _vp, _vp],
     "qk_batch_results": [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp)],
     "qk_batch_completion_ms": [_vp, _P(C.c_double), _i32],
     "qk_batch_key_column": [_vp, _vp, _i32, _i32, _i32, _vp, _vp],

@@ -504,8 +504,9 @@ struct qk_batch_s {
     uint32_t* sk = nullptr;
     int64_t* sv = nullptr;
     int64_t sk_cap = 0;
-    unsigned long long *cnt = nullptr, *sum = nullptr, *mn = nullptr, *mx = nullptr;
+    unsigned long long *cnt = nullptr, *sum = nullptr, *mn = nullptr, *mx = nullptr;  // owned results
     int64_t out_cap = 0;
+    unsigned long long* res[4] = {};  // result arrays of the last run: owned or the caller's
     // pinned copies of the plan and the unit list
     void* host_plan = nullptr;
     BUnit* host_units = nullptr;
@@ -571,6 +572,7 @@ void free_buffers(qk_batch_s* b) {
     b->sk = nullptr;
     b->sv = nullptr;
     b->cnt = b->sum = b->mn = b->mx = nullptr;
+    for (auto& r : b->res) r = nullptr;
     b->unit_cap = b->desc_cap = b->segb_cap = b->sk_cap = b->out_cap = 0;
     if (b->host_units) (void)hipHostFree(b->host_units);
     b->host_units = nullptr;
@@ -600,10 +602,10 @@ BArgs args_of(const qk_batch_s* b) {
     p.segb = b->segb;
     p.sk = b->sk;
     p.sv = b->sv;
-    p.cnt = b->cnt;
-    p.sum = b->sum;
-    p.mn = b->mn;
-    p.mx = b->mx;
+    p.cnt = b->res[0];
+    p.sum = b->res[1];
+    p.mn = b->res[2];
+    p.mx = b->res[3];
     return p;
 }
 
@@ -771,7 +773,9 @@ int64_t estimate_rows(const qk_batch_s* b, const qk_batch_query& x) {
     return (int64_t)rows;
 }
 
-int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s) {
+// ext: four caller arrays (count, sum, min, max) of at least sum(key_mod) entries to write the
+// results to instead of the engine's own arrays, or null.
+int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s, unsigned long long* const* ext) {
     const qk_batch_columns& c = b->cols;
     const qk_batch_tuning& t = b->tune;
     const int64_t nblocks = (c.rows + c.block_rows - 1) / c.block_rows;
@@ -820,7 +824,9 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
     }
     const int32_t nsmall = nq - nlarge;
     // grow the buffers that depend on the batch
-    if (out > b->out_cap) {
+    if (ext) {
+        for (int i = 0; i < 4; i++) b->res[i] = ext[i];
+    } else if (out > b->out_cap) {
         free_graph(b);
         unsigned long long** arrs[] = {&b->cnt, &b->sum, &b->mn, &b->mx};
         const int64_t cap = std::max(out, std::max(b->out_cap + b->out_cap / 2, (int64_t)1 << 20));
@@ -831,6 +837,12 @@ int run_batch(qk_batch_s* b, int32_t nq, const qk_batch_query* q, hipStream_t s)
         b->out_cap = 0;
         for (unsigned long long** a : arrs) QK_TRY(pool_alloc((void**)a, (size_t)cap * 8));
         b->out_cap = cap;
+    }
+    if (!ext) {
+        b->res[0] = b->cnt;
+        b->res[1] = b->sum;
+        b->res[2] = b->mn;
+        b->res[3] = b->mx;
     }
     QK_TRY(ensure_plan(b, nq, std::max(1, nlarge)));
     // estimated passing rows of the large queries, to size the pool and chunks and cut waves
@@ -1009,7 +1021,17 @@ int qk_batch_get_tuning(qk_batch b, qk_batch_tuning* t) {
 int qk_batch_run(qk_batch b, int32_t nq, const qk_batch_query* q, void* stream) {
     if (!b || (nq > 0 && !q) || nq < 0) return fail(QK_ERROR, "qk_batch_run: bad argument");
     QK_TRY(ensure_init());
-    return run_batch(b, nq, q, as_stream(stream));
+    return run_batch(b, nq, q, as_stream(stream), nullptr);
+}
+
+int qk_batch_run_into(qk_batch b, int32_t nq, const qk_batch_query* q, int64_t* count, int64_t* sum,
+                      int64_t* min, int64_t* max, void* stream) {
+    if (!b || (nq > 0 && !q) || nq < 0) return fail(QK_ERROR, "qk_batch_run_into: bad argument");
+    if (nq > 0 && (!count || !sum || !min || !max)) return fail(QK_ERROR, "qk_batch_run_into: null result array");
+    QK_TRY(ensure_init());
+    unsigned long long* ext[4] = {(unsigned long long*)count, (unsigned long long*)sum, (unsigned long long*)min,
+                                  (unsigned long long*)max};
+    return run_batch(b, nq, q, as_stream(stream), ext);
 }
 
 int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_tuning* out, void* stream) {
@@ -1034,10 +1056,10 @@ int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_
     for (const qk_batch_tuning& t : cands) {
         b->tune = t;
         free_graph(b);
-        st = run_batch(b, nq, q, s);  // warm-up (and graph capture)
+        st = run_batch(b, nq, q, s, nullptr);  // warm-up (and graph capture)
         if (st != QK_SUCCESS) break;
         if (hipEventRecord(e0, s) != hipSuccess) { st = QK_RUNTIME_ERROR; break; }
-        st = run_batch(b, nq, q, s);
+        st = run_batch(b, nq, q, s, nullptr);
         if (st != QK_SUCCESS) break;
         if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { st = QK_RUNTIME_ERROR; break; }
         float ms = 0.f;
@@ -1059,10 +1081,10 @@ int qk_batch_autotune(qk_batch b, int32_t nq, const qk_batch_query* q, qk_batch_
 int qk_batch_results(qk_batch b, const int64_t** count, const int64_t** sum, const int64_t** min,
                      const int64_t** max, const int64_t** offsets_host) {
     if (!b) return fail(QK_ERROR, "qk_batch_results: null batch");
-    if (count) *count = (const int64_t*)b->cnt;
-    if (sum) *sum = (const int64_t*)b->sum;
-    if (min) *min = (const int64_t*)b->mn;
-    if (max) *max = (const int64_t*)b->mx;
+    if (count) *count = (const int64_t*)b->res[0];
+    if (sum) *sum = (const int64_t*)b->res[1];
+    if (min) *min = (const int64_t*)b->res[2];
+    if (max) *max = (const int64_t*)b->res[3];
     if (offsets_host) *offsets_host = b->offsets.data();
     return QK_SUCCESS;
 }

@@ -7,13 +7,15 @@
 // released stay under the memory cap, and runs them as one batch.  A new batch is started
 // while another is still on the device, so the two overlap there.  It is started when the
 // device is idle, when a full batch is waiting, or when the oldest request has waited
-// `wait_ms`.  A request is done when its batch's completion event has fired; its groups stay in
-// the engine's result arrays until the caller releases it, and an engine takes a new batch once
-// every request of its last batch is released.
+// `wait_ms`.  A batch writes its groups into a result block of its own; a request is done when
+// its batch's completion event has fired, and the engine is free again at that point.  The
+// block stays in device memory until every request of the batch is released, so results are
+// never held by an engine and callers may release requests in any order.
 //
 // Projected bytes of a request: its dense result table, key_mod groups x 32 B (count, sum, min,
-// max; a dense table has load factor 1), plus, for a request with more than lds_groups groups,
-// its share of the two-pass pool: (estimated rows x 1.25 + 4096) x 12 B.
+// max; a dense table has load factor 1), admitted until the request is released, plus, for a
+// request with more than lds_groups groups, its share of the two-pass pool, (estimated rows x
+// 1.25 + 4096) x 12 B, admitted until its batch finishes.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -32,21 +34,30 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 enum class TState { Pending, Running, Done, Failed };
-enum class EState { Free, Running, Holding };
+enum class EState { Free, Running };
+
+// Device result arrays of one batch, shared by its requests.
+struct Block {
+    void* mem = nullptr;    // count, sum, min, max: 4 x total int64
+    int64_t total = 0;      // sum of key_mod over the batch
+    int32_t refs = 0;       // requests not yet released
+};
 
 struct Ticket {
     qk_batch_query q{};
     int64_t bytes = 0;      // projected
+    int64_t rbytes = 0;     // of which result table (held until release)
     double t_submit = 0.0;  // ms since the server started
     double t_due = 0.0;     // latency is measured from here (the submit time, or a scheduled arrival)
     double t_done = 0.0;
     TState state = TState::Pending;
     int err = QK_SUCCESS;
     std::string msg;
-    int engine = -1;
-    int32_t slot = -1;      // position in its batch
+    Block* blk = nullptr;
+    int64_t off = 0;        // its group 0 in the block's arrays
     bool auto_release = false;
     double* latency_out = nullptr;  // auto-release: where the latency goes
+    bool admitted = false;          // dispatched (its projected bytes were admitted)
 };
 
 struct Engine {
@@ -55,8 +66,7 @@ struct Engine {
     hipEvent_t done = nullptr;
     EState state = EState::Free;
     std::vector<int64_t> tickets;
-    int32_t live = 0;       // tickets of the batch not yet released
-    int64_t bytes = 0;      // projected bytes of the batch
+    int64_t pool_bytes = 0; // projected pool share of the batch (admitted until it finishes)
     double t_start = 0.0;
 };
 
@@ -74,7 +84,9 @@ struct qk_serve_s {
     std::deque<int64_t> pending;
     std::unordered_map<int64_t, Ticket> tickets;
     int64_t next_ticket = 1;
-    int64_t admitted_bytes = 0;       // projected bytes of dispatched, unreleased batches
+    int64_t admitted_bytes = 0;       // projected bytes of dispatched, unreleased requests
+    std::vector<Block*> blocks;       // live result blocks
+    std::vector<void*> to_free;       // device memory of released blocks (freed by the dispatcher)
     int64_t outstanding_auto = 0;     // auto-release tickets not yet done
     bool stop = false;
     std::thread th;
@@ -89,19 +101,35 @@ double now_ms(const qk_serve_s* s) {
     return std::chrono::duration<double, std::milli>(Clock::now() - s->t0).count();
 }
 
+// Drops one reference to a result block (lock held); the last one frees it.
+void unref(qk_serve_s* s, Block* b) {
+    if (!b || --b->refs > 0) return;
+    if (b->mem) s->to_free.push_back(b->mem);
+    s->blocks.erase(std::find(s->blocks.begin(), s->blocks.end(), b));
+    delete b;
+}
+
+// A request leaves the server (lock held): its result table is no longer admitted.
+void drop_ticket(qk_serve_s* s, int64_t id) {
+    Ticket& k = s->tickets[id];
+    if (k.admitted) s->admitted_bytes -= k.rbytes;
+    unref(s, k.blk);
+    s->tickets.erase(id);
+}
+
 // Finishes ticket `id` of a completed batch (lock held).
-void finish(qk_serve_s* s, Engine& e, int64_t id, double t, int err, const std::string& msg) {
+void finish(qk_serve_s* s, int64_t id, double t, int err, const std::string& msg) {
     Ticket& k = s->tickets[id];
     k.t_done = t;
     k.state = err == QK_SUCCESS ? TState::Done : TState::Failed;
     k.err = err;
     k.msg = msg;
+    s->admitted_bytes -= k.bytes - k.rbytes;  // its pool share
     if (err == QK_SUCCESS) s->served++;
     else s->failed++;
     if (k.auto_release) {
         if (k.latency_out) *k.latency_out = err == QK_SUCCESS ? t - k.t_due : -1.0;
-        s->tickets.erase(id);
-        e.live--;
+        drop_ticket(s, id);
         s->outstanding_auto--;
     }
 }
@@ -125,14 +153,6 @@ void drop_pending(qk_serve_s* s) {
     s->pending.clear();
 }
 
-void release_engine_if_idle(qk_serve_s* s, Engine& e) {
-    if (e.state == EState::Holding && e.live == 0) {
-        e.state = EState::Free;
-        s->admitted_bytes -= e.bytes;
-        e.bytes = 0;
-        e.tickets.clear();
-    }
-}
 
 // Marks the batches whose completion event fired (lock held).  Returns true if any did.
 bool collect(qk_serve_s* s) {
@@ -149,12 +169,12 @@ bool collect(qk_serve_s* s) {
             msg = std::string("qk_serve: batch failed on the device: ") + hipGetErrorString(q);
             (void)hipGetLastError();
         }
-        e.state = EState::Holding;
+        e.state = EState::Free;
         s->batches++;
         s->last_batch_ms = t - e.t_start;
         s->busy_ms += s->last_batch_ms;
-        for (int64_t id : e.tickets) finish(s, e, id, t, err, msg);
-        release_engine_if_idle(s, e);
+        for (int64_t id : e.tickets) finish(s, id, t, err, msg);
+        e.tickets.clear();
         any = true;
     }
     return any;
@@ -166,6 +186,8 @@ void dispatcher(qk_serve_s* s) {
     std::vector<qk_batch_query> qs;
     while (true) {
         if (collect(s)) s->cv_done.notify_all();
+        for (void* m : s->to_free) (void)qk::pool_free(m);
+        s->to_free.clear();
         int running = 0;
         Engine* free_e = nullptr;
         for (Engine& e : s->eng) {
@@ -187,30 +209,41 @@ void dispatcher(qk_serve_s* s) {
             if (running == 0 || (int64_t)s->pending.size() >= s->cfg.max_batch || waited >= wait_ms) {
                 // admit the oldest requests while their projected bytes fit
                 Engine& e = *free_e;
-                int64_t bytes = 0;
+                int64_t bytes = 0, pool_bytes = 0, total = 0;
                 e.tickets.clear();
                 qs.clear();
                 while (!s->pending.empty() && (int64_t)qs.size() < s->cfg.max_batch) {
                     Ticket& k = s->tickets[s->pending.front()];
                     if (s->admitted_bytes + bytes + k.bytes > s->cfg.memory_cap) break;
                     bytes += k.bytes;
+                    pool_bytes += k.bytes - k.rbytes;
                     k.state = TState::Running;
-                    k.engine = (int)(&e - s->eng.data());
-                    k.slot = (int32_t)qs.size();
+                    k.admitted = true;
+                    k.off = total;
+                    total += k.q.key_mod;
                     qs.push_back(k.q);
                     e.tickets.push_back(s->pending.front());
                     s->pending.pop_front();
                 }
                 if (!qs.empty()) {
                     e.state = EState::Running;
-                    e.live = (int32_t)qs.size();
-                    e.bytes = bytes;
+                    e.pool_bytes = pool_bytes;
                     e.t_start = t;
                     s->admitted_bytes += bytes;
                     s->peak_bytes = std::max(s->peak_bytes, s->admitted_bytes);
                     s->max_batch_seen = std::max<int64_t>(s->max_batch_seen, (int64_t)qs.size());
+                    Block* blk = new Block();
+                    blk->total = total;
+                    blk->refs = (int32_t)qs.size();
+                    s->blocks.push_back(blk);
+                    for (int64_t id : e.tickets) s->tickets[id].blk = blk;
                     lk.unlock();
-                    int st = qk_batch_run(e.b, (int32_t)qs.size(), qs.data(), e.s);
+                    int st = qk::pool_alloc(&blk->mem, (size_t)total * 32);
+                    if (st == QK_SUCCESS) {
+                        int64_t* r = (int64_t*)blk->mem;
+                        st = qk_batch_run_into(e.b, (int32_t)qs.size(), qs.data(), r, r + total, r + 2 * total,
+                                               r + 3 * total, e.s);
+                    }
                     std::string msg;
                     if (st == QK_SUCCESS && hipEventRecord(e.done, e.s) != hipSuccess) {
                         (void)hipGetLastError();
@@ -222,9 +255,9 @@ void dispatcher(qk_serve_s* s) {
                     lk.lock();
                     if (st != QK_SUCCESS) {
                         const double tf = now_ms(s);
-                        e.state = EState::Holding;
-                        for (int64_t id : e.tickets) finish(s, e, id, tf, st, msg);
-                        release_engine_if_idle(s, e);
+                        e.state = EState::Free;
+                        for (int64_t id : e.tickets) finish(s, id, tf, st, msg);
+                        e.tickets.clear();
                         s->cv_done.notify_all();
                     }
                     launched = true;
@@ -260,6 +293,7 @@ int submit(qk_serve_s* s, const qk_batch_query* q, double due_ms, bool auto_rele
         Ticket& k = s->tickets[id];
         k.q = *q;
         k.bytes = bytes;
+        k.rbytes = (int64_t)q->key_mod * 32;
         k.t_submit = now_ms(s);
         k.t_due = due_ms >= 0 ? due_ms : k.t_submit;
         k.auto_release = auto_release;
@@ -281,12 +315,19 @@ void destroy(qk_serve_s* s) {
         s->cv_work.notify_all();
         s->th.join();
     }
+    (void)hipSetDevice(s->device);
     for (Engine& e : s->eng) {
         if (e.s) (void)hipStreamSynchronize(e.s);
         if (e.b) (void)qk_batch_destroy(e.b);
         if (e.done) (void)hipEventDestroy(e.done);
         if (e.s) (void)hipStreamDestroy(e.s);
     }
+    // result blocks of requests never released
+    for (Block* b : s->blocks) {
+        if (b->mem) s->to_free.push_back(b->mem);
+        delete b;
+    }
+    for (void* m : s->to_free) (void)qk::pool_free(m);
     delete s;
 }
 
@@ -376,13 +417,12 @@ int qk_serve_result(qk_serve s, int64_t ticket, const int64_t** count, const int
     if (it == s->tickets.end() || it->second.state != TState::Done)
         return qk::fail(QK_ERROR, "qk_serve_result: ticket %lld is not done", (long long)ticket);
     const Ticket& k = it->second;
-    const int64_t *c = nullptr, *sm = nullptr, *mn = nullptr, *mx = nullptr, *off = nullptr;
-    QK_TRY(qk_batch_results(s->eng[(size_t)k.engine].b, &c, &sm, &mn, &mx, &off));
-    const int64_t o = off[k.slot];
-    if (count) *count = c + o;
-    if (sum) *sum = sm + o;
-    if (min) *min = mn + o;
-    if (max) *max = mx + o;
+    const int64_t* r = (const int64_t*)k.blk->mem;
+    const int64_t t = k.blk->total;
+    if (count) *count = r + k.off;
+    if (sum) *sum = r + t + k.off;
+    if (min) *min = r + 2 * t + k.off;
+    if (max) *max = r + 3 * t + k.off;
     return QK_SUCCESS;
 }
 
@@ -407,12 +447,10 @@ int qk_serve_release(qk_serve s, int64_t ticket) {
         if (k.state == TState::Running) return qk::fail(QK_ERROR, "qk_serve_release: ticket is running");
         if (k.state == TState::Pending) {
             s->pending.erase(std::find(s->pending.begin(), s->pending.end(), ticket));
+            s->tickets.erase(it);
         } else {
-            Engine& e = s->eng[(size_t)k.engine];
-            e.live--;
-            release_engine_if_idle(s, e);
+            drop_ticket(s, ticket);
         }
-        s->tickets.erase(it);
     }
     s->cv_work.notify_one();
     return QK_SUCCESS;

@@ -260,29 +260,26 @@ __host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, int32_t kma
     return b;
 }
 
-// The two candidate buckets of a key (independent halves of its hash).
-__device__ __forceinline__ void buckets_of(uint64_t h, uint32_t nb, uint32_t& b1, uint32_t& b2) {
-    b1 = (uint32_t)(((h >> 32) * (uint64_t)nb) >> 32);
-    b2 = (uint32_t)(((h & 0xffffffffull) * (uint64_t)nb) >> 32);
-    if (b2 == b1) b2 = (b1 + 1 == nb) ? 0u : b1 + 1;
+// The bucket of a key in the heavy table: a multiplicative hash of the key's two halves folded
+// together (Knuth's golden-ratio multiplier; the high bits of the product pick the bucket).
+// The table only needs keys spread over the buckets, so this stands in for the full 64-bit
+// hash on the path every row takes.
+__device__ __forceinline__ uint32_t bucket_of(uint64_t k, uint32_t nb) {
+    const uint32_t f = ((uint32_t)k ^ (uint32_t)(k >> 32)) * 0x9e3779b1u;
+    return __umulhi(f, nb);
 }
 
-// Slot of key k in the heavy table, or -1 (two buckets of kBucket keys, read as 128-bit words).
-__device__ __forceinline__ int heavy_find(const uint64_t* lkey, uint32_t nb, uint64_t k, uint64_t h) {
-    uint32_t b1, b2;
-    buckets_of(h, nb, b1, b2);
-    const ulonglong2* p1 = reinterpret_cast<const ulonglong2*>(lkey + (size_t)b1 * kBucket);
-    const ulonglong2* p2 = reinterpret_cast<const ulonglong2*>(lkey + (size_t)b2 * kBucket);
-    const ulonglong2 a0 = p1[0], a1 = p1[1], c0 = p2[0], c1 = p2[1];
+// Slot of key k in the heavy table, or -1 (one bucket of kBucket keys, two 128-bit reads).
+__device__ __forceinline__ int heavy_find(const uint64_t* lkey, uint32_t nb, uint64_t k) {
+    const uint32_t b = bucket_of(k, nb);
+    const ulonglong2* p = reinterpret_cast<const ulonglong2*>(lkey + (size_t)b * kBucket);
+    const ulonglong2 a0 = p[0], a1 = p[1];
+    const int base = (int)(b * kBucket);
     int s = -1;
-    if (a0.x == k) s = (int)(b1 * kBucket + 0);
-    if (a0.y == k) s = (int)(b1 * kBucket + 1);
-    if (a1.x == k) s = (int)(b1 * kBucket + 2);
-    if (a1.y == k) s = (int)(b1 * kBucket + 3);
-    if (c0.x == k) s = (int)(b2 * kBucket + 0);
-    if (c0.y == k) s = (int)(b2 * kBucket + 1);
-    if (c1.x == k) s = (int)(b2 * kBucket + 2);
-    if (c1.y == k) s = (int)(b2 * kBucket + 3);
+    s = a0.x == k ? base : s;
+    s = a0.y == k ? base + 1 : s;
+    s = a1.x == k ? base + 2 : s;
+    s = a1.y == k ? base + 3 : s;
     return s;
 }
 
@@ -342,10 +339,10 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     for (int q = tid; q < a.P; q += bd) cursor[q] = a.accumulate ? a.r_count[region0 + q] : 0u;
     if (tid == 0) nnew[0] = 0;
     __syncthreads();
-    // Place each heavy key in the emptier of its two buckets, the most frequent keys first
-    // (one round per power of two of the sample count), so that the keys carrying most rows
-    // always find a slot.  A key that finds both buckets full stays cold: its rows then go
-    // through its partition like any other key's.
+    // Place each heavy key in a free slot of its bucket, the most frequent keys first (one
+    // round per power of two of the sample count), so that the keys carrying most rows always
+    // find a slot.  A key that finds its bucket full stays cold: its rows then go through its
+    // partition like any other key's.
     const int32_t K = min(a.meta[0], a.kmax);
     // kmax <= 4 * blockDim, so each thread holds at most kPlace heavy keys
     constexpr int kPlace = 4;
@@ -370,22 +367,12 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
 #pragma unroll
         for (int i = 0; i < kPlace; i++) {
             if (pcls[i] != cls) continue;
-            uint32_t b1, b2;
-            buckets_of(hash64(pkey[i]), nb, b1, b2);
+            const uint32_t bb = bucket_of(pkey[i], nb);
             int placed = -1;
-            for (int attempt = 0; attempt < 2 * kBucket && placed < 0; attempt++) {
-                int f1 = 0, f2 = 0;
-                for (int j = 0; j < kBucket; j++) {
-                    f1 += lkey[b1 * kBucket + j] != kEmpty;
-                    f2 += lkey[b2 * kBucket + j] != kEmpty;
-                }
-                const uint32_t bb = (f1 <= f2) ? b1 : b2;
-                if (((f1 <= f2) ? f1 : f2) >= kBucket) break;
-                for (int j = 0; j < kBucket && placed < 0; j++) {
-                    unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
-                                                       (unsigned long long)kEmpty, (unsigned long long)pkey[i]);
-                    if (old == kEmpty) placed = (int)(bb * kBucket + j);
-                }
+            for (int j = 0; j < kBucket && placed < 0; j++) {
+                const unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
+                                                         (unsigned long long)kEmpty, (unsigned long long)pkey[i]);
+                if (old == kEmpty) placed = (int)(bb * kBucket + j);
             }
             slot_of[tid + i * bd] = placed;
         }
@@ -421,13 +408,15 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             t.kn[k] = false;
             if constexpr (kTypedKey<KT>) {
                 t.kword[k] = kbits[(r0 >> 6) * kstep];
-                t.key[k] = (uint64_t)(int64_t)((const KT*)kc.col[0])[r];
+                // streamed once: non-temporal, so that the lines of the cold regions being
+                // filled stay in the L2 instead of being pushed out by the column reads
+                t.key[k] = (uint64_t)(int64_t)__builtin_nontemporal_load((const KT*)kc.col[0] + r);
             } else {
                 uint64_t kk[1];
                 load_key<1>(ks, kc, r, kk, t.kn[k]);
                 t.key[k] = kk[0];
             }
-            if constexpr (HAS_V) t.val[k] = vals[r]; else t.val[k] = 0;
+            if constexpr (HAS_V) t.val[k] = __builtin_nontemporal_load(vals + r); else t.val[k] = 0;
         }
     };
     // A tile's rows after the WHERE: key (0 for a NULL key), value, and per step whether the
@@ -463,14 +452,13 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             if (!((o.pass >> k) & 1)) continue;
             const bool vn = (o.vnull >> k) & 1, kn = (o.knull >> k) & 1;
             const uint64_t kk = o.key[k];
-            const uint64_t h = hash64(kk);
             int ls;
             if (kn)
                 ls = H;
             else if (kk == kEmpty)
                 ls = H + 1;
             else
-                ls = heavy_find(lkey, nb, kk, h);
+                ls = heavy_find(lkey, nb, kk);
             Part pr;
             if constexpr (HAS_V) pr = row_part<VT>(o.val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
             if (ls >= 0) {
@@ -489,6 +477,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                 }
                 continue;
             }
+            const uint64_t h = hash64(kk);
             if (!vn) {
                 // part_shift is 56..64; the split shift gives 0 for 64 without a select
                 const uint32_t q = (uint32_t)((h >> 1) >> (a.part_shift - 1));

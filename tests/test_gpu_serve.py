@@ -90,7 +90,7 @@ def test_results_same_alone_and_in_the_mix(table):
         for t in th:
             t.join()
         for i, t in enumerate(tickets):
-            srv.wait(t)
+            srv.wait(t, timeout_ms=120_000)
             assert srv.result_host(t, qs[i].key_mod) == ref[i], f"query {i} differs in the mix"
             assert srv.latency_ms(t) >= 0
             srv.release(t)
@@ -98,6 +98,24 @@ def test_results_same_alone_and_in_the_mix(table):
         assert st["served"] == len(qs) and st["failed"] == 0 and st["pending"] == 0
         assert st["batches"] >= len(qs) // 16 and st["max_batch"] <= 16
         assert st["admitted_bytes"] == 0
+
+
+def test_release_in_any_order(table):
+    """Results live in per-batch blocks, not in the engines: waiting for and releasing requests
+    against the batch order (the last submitted first) completes, with few engines and small
+    batches, and every result stays intact until its own release."""
+    d, cols = table
+    qs = make_queries(24, seed=8)
+    ref = alone(cols, qs)
+    with S.Server(*ptrs(cols), N, BLOCK, B_RANGE, S.Config(engines=2, max_batch=3)) as srv:
+        tickets = [srv.submit(q) for q in qs]
+        for i in reversed(range(len(qs))):
+            srv.wait(tickets[i], timeout_ms=120_000)
+        for i in reversed(range(len(qs))):
+            assert srv.result_host(tickets[i], qs[i].key_mod) == ref[i], f"query {i}"
+            srv.release(tickets[i])
+        st = srv.stats()
+        assert st["served"] == len(qs) and st["admitted_bytes"] == 0 and st["batches"] >= 8
 
 
 def test_memory_cap_limits_admission(table):
@@ -109,7 +127,7 @@ def test_memory_cap_limits_admission(table):
     with S.Server(*ptrs(cols), N, BLOCK, B_RANGE, S.Config(engines=3, memory_cap=cap)) as srv:
         ts = [srv.submit(q) for q in qs]
         for i, t in enumerate(ts):
-            srv.wait(t)
+            srv.wait(t, timeout_ms=120_000)
             assert srv.result_host(t, qs[i].key_mod) == ref[i]
             srv.release(t)
         st = srv.stats()
