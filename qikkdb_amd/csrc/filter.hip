@@ -268,23 +268,10 @@ void split(const Bound<T>& b, T& x, T& i) {
 
 }  // namespace
 
-bool fold_value_range(PredSet* ps, uint32_t vpred, int32_t vtype) {
-    if (ps->n == 0) return false;
-    Fold f;
-    for (int i = 0; i < ps->n; i++) {
-        const qk_predicate& p = ps->p[i];
-        if (!((vpred >> i) & 1u) || p.rhs_kind != QK_CONST) return false;
-        if (p.op != QK_GREATER && p.op != QK_GREATER_EQUAL && p.op != QK_LESS &&
-            p.op != QK_LESS_EQUAL && p.op != QK_EQUAL)
-            return false;
-        switch (vtype) {
-        case QK_INT: fold_typed<int32_t>(p, f); break;
-        case QK_LONG: fold_typed<int64_t>(p, f); break;
-        case QK_FLOAT: fold_typed<float>(p, f); break;
-        case QK_DOUBLE: fold_typed<double>(p, f); break;
-        default: return false;
-        }
-    }
+namespace {
+
+// The interval the kernels test, from the folded bounds.
+ValueRange range_of(const Fold& f) {
     ValueRange vr{};
     vr.doms = f.doms;
     // integer bounds inclusive; a strict bound at the end of the range empties the interval
@@ -300,7 +287,49 @@ bool fold_value_range(PredSet* ps, uint32_t vpred, int32_t vtype) {
     split(f.hi_f, vr.hi_fx, vr.hi_fi);
     split(f.lo_d, vr.lo_dx, vr.lo_di);
     split(f.hi_d, vr.hi_dx, vr.hi_di);
-    ps->vr = vr;
+    return vr;
+}
+
+bool foldable_op(int32_t op) {
+    return op == QK_GREATER || op == QK_GREATER_EQUAL || op == QK_LESS || op == QK_LESS_EQUAL || op == QK_EQUAL;
+}
+
+}  // namespace
+
+bool fold_value_range(PredSet* ps, uint32_t vpred, int32_t vtype) {
+    if (ps->n == 0) return false;
+    Fold f;
+    for (int i = 0; i < ps->n; i++) {
+        const qk_predicate& p = ps->p[i];
+        if (!((vpred >> i) & 1u) || p.rhs_kind != QK_CONST || !foldable_op(p.op)) return false;
+        switch (vtype) {
+        case QK_INT: fold_typed<int32_t>(p, f); break;
+        case QK_LONG: fold_typed<int64_t>(p, f); break;
+        case QK_FLOAT: fold_typed<float>(p, f); break;
+        case QK_DOUBLE: fold_typed<double>(p, f); break;
+        default: return false;
+        }
+    }
+    ps->vr = range_of(f);
+    return true;
+}
+
+bool fold_column_range(PredSet* ps, uint32_t vpred) {
+    if (ps->n == 0 || vpred != 0) return false;
+    const qk_predicate& p0 = ps->p[0];
+    if ((p0.type != QK_INT && p0.type != QK_LONG) || !p0.column) return false;
+    Fold f;
+    for (int i = 0; i < ps->n; i++) {
+        const qk_predicate& p = ps->p[i];
+        if (p.column != p0.column || p.type != p0.type || p.null_mask != p0.null_mask || p.rhs_kind != QK_CONST ||
+            !foldable_op(p.op))
+            return false;
+        if (p0.type == QK_INT) fold_typed<int32_t>(p, f); else fold_typed<int64_t>(p, f);
+    }
+    ps->vr = range_of(f);
+    ps->rcol = p0.column;
+    ps->rnull = p0.null_mask;
+    ps->rwide = p0.type == QK_LONG ? 1 : 0;
     return true;
 }
 

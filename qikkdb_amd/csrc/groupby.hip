@@ -158,6 +158,8 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
                 if constexpr (HAS_V) val[k] = vals[r];
                 vn[k] = vnull && null_bit(vnull, r);
                 if (MODE == 1) act[k] = mask[r] != 0;
+                if (MODE == 4)
+                    act[k] = !(preds.rnull && null_bit(preds.rnull, r)) && col_in_range(preds, range_col(preds, r));
             }
         }
         if constexpr (MODE == 3 && HAS_V) {
@@ -463,6 +465,7 @@ int launch_rows_m(int mode, const GView& g, const KeySpec& ks, const KeyCols& kc
     case 0: return launch_rows_t<KW, VT, 0>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
     case 1: return launch_rows_t<KW, VT, 1>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
     case 2: return launch_rows_t<KW, VT, 2>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
+    case 4: return launch_rows_t<KW, VT, 4>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
     default:
         if constexpr (VTraits<VT>::has)
             return launch_rows_t<KW, VT, 3>(g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, H, block, wpc, s);
@@ -649,6 +652,7 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
             vpred |= 1u << p;
     // WHERE on the value column alone: test one interval per row instead of the list
     if (mode == 2 && values && fold_value_range(&ps, vpred, h->value_type)) mode = 3;
+    else if (mode == 2 && fold_column_range(&ps, vpred)) mode = 4;
     hipStream_t s = as_stream(stream);
     GView g = view_of(h);
     std::pair<hipEvent_t, hipEvent_t>* ev = h->profiling ? next_events(h) : nullptr;

@@ -119,7 +119,22 @@ struct PredSet {
     int32_t n;
     qk_predicate p[kMaxPreds];
     ValueRange vr;  // the predicates folded, when they all fold (see fold_value_range)
+    // fold_column_range: the one integer column every predicate compares
+    const void* rcol;
+    const uint64_t* rnull;
+    int32_t rwide;  // 1: int64, 0: int32
 };
+
+// The folded range column at row r, widened to int64.
+__device__ __forceinline__ int64_t range_col(const PredSet& ps, int64_t r) {
+    return ps.rwide ? __builtin_nontemporal_load((const int64_t*)ps.rcol + r)
+                    : (int64_t)__builtin_nontemporal_load((const int32_t*)ps.rcol + r);
+}
+
+// Whether a non-NULL value of the range column lies in the folded interval.
+__device__ __forceinline__ bool col_in_range(const PredSet& ps, int64_t c) {
+    return ps.rwide ? in_range<int64_t>(ps.vr, c) : in_range<int32_t>(ps.vr, (int32_t)c);
+}
 
 // Host-side validation of a predicate array; returns QK_SUCCESS or an error.
 int check_preds(int32_t npreds, const qk_predicate* preds, PredSet* out);
@@ -128,5 +143,11 @@ int check_preds(int32_t npreds, const qk_predicate* preds, PredSet* out);
 // per such predicate) of type vtype with a constant by >, >=, <, <= or ==.  Returns whether
 // it did; the fused kernels then test the interval instead of the predicate list.
 bool fold_value_range(PredSet* ps, uint32_t vpred, int32_t vtype);
+
+// Fold the predicates into ps->vr / rcol / rnull / rwide when none reads the value column
+// (vpred == 0) and every one compares the same int32 or int64 column (same null mask) with a
+// constant by >, >=, <, <= or ==.  Returns whether it did; the fused kernels then load that
+// column with the rest of a tile and test the interval (a NULL value fails it).
+bool fold_column_range(PredSet* ps, uint32_t vpred);
 
 }  // namespace qk

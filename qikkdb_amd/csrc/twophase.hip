@@ -19,12 +19,15 @@ constexpr int kBlock = 1024;
 constexpr int kBucket = 4;    // slots per bucket of the phase-1 heavy table
 constexpr int kSampleLds = 2048;
 
-// VV is VT, or a placeholder type when the table reads no value column.
+// VV is VT, or a placeholder type when the table reads no value column.  MODE: 0 no WHERE,
+// 1 row mask, 2 predicate list, 3 folded interval on the value column, 4 folded interval on
+// another integer column, whose value (cv) and NULL bit (cn) the caller loaded.
 template <typename VT, int MODE, typename VV>
 __device__ __forceinline__ bool row_passes(const PredSet& preds, uint32_t vpred, const int8_t* mask,
-                                           int64_t r, const VV& v, bool vn) {
+                                           int64_t r, const VV& v, bool vn, int64_t cv = 0, bool cn = false) {
     if (MODE == 1) return mask[r] != 0;
     if constexpr (MODE == 3 && VTraits<VT>::has) return !vn && in_range<VT>(preds.vr, v);
+    if (MODE == 4) return !cn && col_in_range(preds, cv);
     if (MODE == 2) {
         for (int p = 0; p < preds.n; p++) {
             const qk_predicate& q = preds.p[p];
@@ -118,7 +121,13 @@ __global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const
         typename std::conditional<VTraits<VT>::has, VT, int>::type v{};
         if constexpr (VTraits<VT>::has) v = vals[r];
         const bool vn = vnull && null_bit(vnull, r);
-        if (!row_passes<VT, MODE>(preds, vpred, mask, r, v, vn)) continue;
+        int64_t cv = 0;
+        bool cn = false;
+        if (MODE == 4) {
+            cv = range_col(preds, r);
+            cn = preds.rnull && null_bit(preds.rnull, r);
+        }
+        if (!row_passes<VT, MODE>(preds, vpred, mask, r, v, vn, cv, cn)) continue;
         uint64_t k[1];
         bool kn;
         load_key<1>(ks, kc, r, k, kn);
@@ -393,8 +402,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     const uint64_t* kbits = kc.nm[0] ? kc.nm[0] : a.zero;
     const int64_t kstep = kc.nm[0] ? 1 : 0;
     using VV = typename std::conditional<HAS_V, VT, int>::type;
+    const uint64_t* cbits = preds.rnull ? preds.rnull : a.zero;
+    const int64_t cstep = preds.rnull ? 1 : 0;
     struct Tile {
-        uint64_t key[kP1Steps], vword[kP1Steps], kword[kP1Steps];
+        uint64_t key[kP1Steps], vword[kP1Steps], kword[kP1Steps], cword[kP1Steps];
+        int64_t cv[kP1Steps];
         bool kn[kP1Steps];
         VV val[kP1Steps];
     };
@@ -417,6 +429,12 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                 t.key[k] = kk[0];
             }
             if constexpr (HAS_V) t.val[k] = __builtin_nontemporal_load(vals + r); else t.val[k] = 0;
+            t.cv[k] = 0;
+            t.cword[k] = 0;
+            if (MODE == 4) {
+                t.cv[k] = range_col(preds, r);
+                t.cword[k] = cbits[(r0 >> 6) * cstep];
+            }
         }
     };
     // A tile's rows after the WHERE: key (0 for a NULL key), value, and per step whether the
@@ -440,7 +458,8 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             }
             o.key[k] = kk;
             o.val[k] = t.val[k];
-            const bool pass = r < n && row_passes<VT, MODE>(preds, vpred, mask, r, t.val[k], vn);
+            const bool pass = r < n && row_passes<VT, MODE>(preds, vpred, mask, r, t.val[k], vn, t.cv[k],
+                                                            (t.cword[k] >> lane) & 1);
             o.pass |= (uint32_t)pass << k;
             o.vnull |= (uint32_t)vn << k;
             o.knull |= (uint32_t)kn << k;
@@ -606,38 +625,29 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     }
     __syncthreads();
 
-    // A new key takes a slot only while fewer than `limit` slots are in use (reserved before
-    // the claim, returned if the key turns out to be there already); past that the caller
-    // aggregates the row straight into the global table.  Linear probing without deletion:
-    // once an empty slot is seen, the key is not further along the chain.
-    const uint32_t limit = (uint32_t)((int64_t)H * 4 / 5);
+    // A new key takes a slot only while fewer than `limit` slots are in use; past that the
+    // caller aggregates the row straight into the global table.  The count is read before the
+    // claim and raised after it, so up to one claim per thread may pass the check together;
+    // limit <= H - blockDim keeps empty slots in the table then (and a probe gives up after H
+    // slots in any case).  Linear probing without deletion: once an empty slot is seen, the key
+    // is not further along the chain.
+    const uint32_t limit = (uint32_t)max(H / 2, min(H * 4 / 5, H - bd));
     auto insert_from = [&](uint64_t key, uint32_t s) -> int {
-        bool reserved = false;
         for (int it = 0; it < H; it++) {
             const uint64_t cur = tkey[s];
-            if (cur == key) {
-                if (reserved) atomicSub(&misc[0], 1u);
-                return (int)s;
-            }
+            if (cur == key) return (int)s;
             if (cur == kEmpty) {
-                if (!reserved) {
-                    if (atomicAdd(&misc[0], 1u) >= limit) {
-                        atomicSub(&misc[0], 1u);
-                        return -1;
-                    }
-                    reserved = true;
-                }
+                if (__hip_atomic_load(&misc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= limit) return -1;
                 const unsigned long long old = atomicCAS((unsigned long long*)&tkey[s],
                                                          (unsigned long long)kEmpty, (unsigned long long)key);
-                if (old == kEmpty) return (int)s;
-                if (old == key) {
-                    atomicSub(&misc[0], 1u);
+                if (old == kEmpty) {
+                    atomicAdd(&misc[0], 1u);
                     return (int)s;
                 }
+                if (old == key) return (int)s;
             }
             s = (s + 1 == (uint32_t)H) ? 0u : s + 1;
         }
-        if (reserved) atomicSub(&misc[0], 1u);
         return -1;
     };
     auto insert = [&](uint64_t key) -> int { return insert_from(key, p2_home(hash64(key), H)); };
@@ -778,13 +788,16 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 key[i] = a.r_keys[region + j];
                 if constexpr (HAS_V) val[i] = ((const VT*)a.r_vals)[region + j]; else val[i] = 0;
             }
-            // first probe of every row at once; most rows find their key there
+            // first two probes of every row at once; most rows find their key there
             uint32_t home[kP2Rows];
-            uint64_t first[kP2Rows];
+            uint64_t first[kP2Rows], second[kP2Rows];
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) home[i] = p2_home(hash64(key[i]), H);
 #pragma unroll
-            for (int i = 0; i < kP2Rows; i++) first[i] = tkey[home[i]];
+            for (int i = 0; i < kP2Rows; i++) {
+                first[i] = tkey[home[i]];
+                second[i] = tkey[home[i] + 1 == (uint32_t)H ? 0u : home[i] + 1];
+            }
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 if (j0 + i * 64 + lane >= cnt) continue;
@@ -794,7 +807,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 } else {
                     pr = row_part<NoValue>(NoValue{}, false);
                 }
-                const int s = first[i] == key[i] ? (int)home[i] : insert_from(key[i], home[i]);
+                const uint32_t next = home[i] + 1 == (uint32_t)H ? 0u : home[i] + 1;
+                int s;
+                if (first[i] == key[i]) s = (int)home[i];
+                else if (second[i] == key[i] && first[i] != kEmpty) s = (int)next;
+                else s = insert_from(key[i], home[i]);
                 if (s >= 0)
                     add(s, 0, 1, pr.sum, pr.mn, pr.mx);
                 else
@@ -841,6 +858,7 @@ int launch_sample_m(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols&
     case 0: return launch_sample<VT, 0>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
     case 1: return launch_sample<VT, 1>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
     case 2: return launch_sample<VT, 2>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
+    case 4: return launch_sample<VT, 4>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
     default:
         if constexpr (VTraits<VT>::has)
             return launch_sample<VT, 3>(b, ks, kc, vals, vnull, mask, ps, vpred, n, ns, st, sm, s);
@@ -887,6 +905,7 @@ int launch_p1_m(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, co
     case 0: return launch_p1<VT, 0, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     case 1: return launch_p1<VT, 1, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     case 2: return launch_p1<VT, 2, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 4: return launch_p1<VT, 4, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     default:
         if constexpr (VTraits<VT>::has)
             return launch_p1<VT, 3, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);

@@ -418,6 +418,30 @@ def fold_values(vt, n, rng):
     return v.astype(np.float64)
 
 
+COLUMN_FOLD_CASES = [c for c in FOLD_CASES if c[0] in (np.int32, np.int64)]
+
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("case", range(len(COLUMN_FOLD_CASES)))
+def test_column_range_fold_matches_predicates(case, path):
+    """WHERE on another integer column than the aggregated one (the folded column interval):
+    same groups as the predicates evaluated one by one, NULLs of that column failing them."""
+    ct, preds = COLUMN_FOLD_CASES[case]
+    rng = np.random.default_rng(700 + case)
+    n = 300_000
+    keys = rng.integers(0, 5000, n).astype(np.int64)
+    vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    filt = fold_values(ct, n, rng)
+    fnull = rng.random(n) < 0.05
+    w = [Where(op, "filter", c, t) for op, c, t in preds]
+    aggs = [qk.AGG_SUM, qk.AGG_COUNT, qk.AGG_MIN, qk.AGG_MAX]
+    extra = {"filter": (filt, fnull)}
+    gpu = gpu_group_by(aggs, [keys], vals, None, None, where=w, capacity=10_000, path=path, block=100_000,
+                       extra_cols=extra)
+    cpu = cpu_group_by(aggs, [keys], vals, None, None, where=w, extra_cols=extra)
+    compare(gpu, cpu, len(aggs))
+
+
 @pytest.mark.parametrize("path", [0, 1])
 @pytest.mark.parametrize("case", range(len(FOLD_CASES)))
 def test_value_range_fold_matches_predicates(case, path):
