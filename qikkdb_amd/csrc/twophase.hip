@@ -19,6 +19,33 @@ constexpr int kBlock = 1024;
 constexpr int kBucket = 4;    // slots per bucket of the phase-1 heavy table
 constexpr int kSampleLds = 2048;
 
+// A cold row in its region: the key and the value packed into one record of 32-bit words
+// (8, 12 or 16 bytes), so a workgroup fills one line per partition at a time instead of one
+// line per column; fewer lines are half-written when the L2 evicts them.
+template <typename VT>
+struct RecOf {
+    static constexpr int words = 2 + (VTraits<VT>::has ? (int)(sizeof(VT) / 4) : 0);
+    struct alignas(4) Rec { uint32_t w[words]; };
+};
+
+template <typename VT, typename VV>
+__device__ __forceinline__ void put_rec(uint32_t* base, int64_t at, uint64_t key, const VV& v) {
+    using R = RecOf<VT>;
+    typename R::Rec r;
+    r.w[0] = (uint32_t)key;
+    r.w[1] = (uint32_t)(key >> 32);
+    if constexpr (VTraits<VT>::has) __builtin_memcpy(&r.w[2], &v, sizeof(VT));
+    *reinterpret_cast<typename R::Rec*>(base + at * R::words) = r;
+}
+
+template <typename VT, typename VV>
+__device__ __forceinline__ void get_rec(const uint32_t* base, int64_t at, uint64_t& key, VV& v) {
+    using R = RecOf<VT>;
+    const typename R::Rec r = *reinterpret_cast<const typename R::Rec*>(base + at * R::words);
+    key = (uint64_t)r.w[0] | ((uint64_t)r.w[1] << 32);
+    if constexpr (VTraits<VT>::has) __builtin_memcpy(&v, &r.w[2], sizeof(VT)); else v = 0;
+}
+
 // VV is VT, or a placeholder type when the table reads no value column.  MODE: 0 no WHERE,
 // 1 row mask, 2 predicate list, 3 folded interval on the value column, 4 folded interval on
 // another integer column, whose value (cv) and NULL bit (cn) the caller loaded.
@@ -250,8 +277,7 @@ struct P1Args {
     uint64_t* p_sum;
     uint64_t* p_min;
     uint64_t* p_max;
-    uint64_t* r_keys;
-    void* r_vals;
+    uint32_t* r_rec;       // cold-row records [W][P][cap] (RecOf<VT>)
     uint32_t* r_count;
     const uint64_t* zero;  // a zero word (stands in for an absent null bitmap)
     int32_t part_shift;    // partition of a key hash: h >> part_shift (0 when >= 64)
@@ -503,8 +529,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                 const uint32_t c = atomicAdd(&cursor[q], 1u);
                 if (c < (uint32_t)a.cap) {
                     const int64_t at = (region0 + q) * (int64_t)a.cap + c;
-                    a.r_keys[at] = kk;
-                    if constexpr (HAS_V) ((VT*)a.r_vals)[at] = o.val[k];
+                    put_rec<VT>(a.r_rec, at, kk, o.val[k]);
                     continue;
                 }
             }
@@ -568,8 +593,7 @@ struct P2Args {
     const uint64_t* p_sum;
     const uint64_t* p_min;
     const uint64_t* p_max;
-    const uint64_t* r_keys;
-    const void* r_vals;
+    const uint32_t* r_rec;
     const uint32_t* r_count;
 };
 
@@ -785,8 +809,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 const int32_t j = min(j0 + i * 64 + lane, cnt - 1);
-                key[i] = a.r_keys[region + j];
-                if constexpr (HAS_V) val[i] = ((const VT*)a.r_vals)[region + j]; else val[i] = 0;
+                get_rec<VT>(a.r_rec, region + j, key[i], val[i]);
             }
             // first two probes of every row at once; most rows find their key there
             uint32_t home[kP2Rows];
@@ -884,8 +907,7 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
     a.p_sum = b.p_sum;
     a.p_min = b.p_min;
     a.p_max = b.p_max;
-    a.r_keys = b.r_keys;
-    a.r_vals = b.r_vals;
+    a.r_rec = b.r_rec;
     a.r_count = b.r_count;
     a.zero = b.zero_word;
     a.part_shift = g.part_shift;
@@ -948,8 +970,7 @@ int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
     a.p_sum = b.p_sum;
     a.p_min = b.p_min;
     a.p_max = b.p_max;
-    a.r_keys = b.r_keys;
-    a.r_vals = b.r_vals;
+    a.r_rec = b.r_rec;
     a.r_count = b.r_count;
     if ((fields & F_SUM) && !(fields & (F_MIN | F_MAX)))
         k_tp_phase2<VT, kFieldsSum><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
@@ -968,7 +989,7 @@ bool twophase_supported(int32_t kw, int32_t value_type) {
 
 void twophase_free(TwoPhaseBufs& b) {
     void* ptrs[] = {b.heavy_keys, b.heavy_cnt, b.cand_cnt, b.heavy_meta, b.sample_keys, b.sample_cnt, b.hist, b.cand,
-                    b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_keys, b.r_vals, b.r_count,
+                    b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_rec, b.r_count,
                     b.zero_word, b.gview};
     for (void* p : ptrs)
         if (p) pool_free(p);
@@ -991,7 +1012,7 @@ int64_t region_rows_for(const TwoPhaseBufs& b, int64_t n, bool defer) {
 bool twophase_needs_realloc(const TwoPhaseBufs& b, int32_t parts, int32_t value_type,
                             bool has_values, int64_t n, bool defer) {
     const int vbytes = has_values ? type_size(value_type) : 0;
-    return !b.r_keys || b.block_alloc < n || b.vbytes != vbytes || b.P != parts ||
+    return !b.r_rec || b.block_alloc < n || b.vbytes != vbytes || b.P != parts ||
            b.rows_alloc < region_rows_for(b, n, defer);
 }
 
@@ -1047,8 +1068,7 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     if (fields & F_SUM) A((void**)&b.p_sum, (size_t)b.W * kp * 8);
     if (fields & F_MIN) A((void**)&b.p_min, (size_t)b.W * kp * 8);
     if (fields & F_MAX) A((void**)&b.p_max, (size_t)b.W * kp * 8);
-    A((void**)&b.r_keys, (size_t)regions * 8);
-    if (vbytes) A((void**)&b.r_vals, (size_t)regions * vbytes);
+    A((void**)&b.r_rec, (size_t)regions * (8 + (size_t)vbytes));
     A((void**)&b.r_count, (size_t)b.W * parts * 4);
     A((void**)&b.zero_word, 64);
     A((void**)&b.gview, sizeof(GView));

@@ -49,8 +49,7 @@ struct TwoPhaseBufs {
     uint64_t* p_min = nullptr;
     uint64_t* p_max = nullptr;
     // cold regions [W][P][cap]
-    uint64_t* r_keys = nullptr;
-    void* r_vals = nullptr;
+    uint32_t* r_rec = nullptr;        // key + value records
     uint32_t* r_count = nullptr;      // [W][P]
     uint64_t* zero_word = nullptr;    // 64 zero bytes
     GView* gview = nullptr;           // device copy of the table view (phase 1 reads it there)
