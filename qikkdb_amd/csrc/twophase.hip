@@ -781,12 +781,14 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         }
     }
 
-    // Cold rows of this partition: in each round every wave takes the partition's region of
-    // one phase-1 workgroup and walks it 64 x kP2Rows rows at a time, loading them all before
-    // inserting any.  A round in which many rows found no LDS slot means the partition holds
-    // more keys than the table: the table is then merged and cleared before the next round.
+    // Cold rows of this partition: in each round every wave takes the partition's regions of
+    // kRegionsPerWave phase-1 workgroups and walks them as one run of rows, 64 x kP2Rows rows
+    // at a time, loading them all before inserting any.  A round in which many rows found no
+    // LDS slot means the partition holds more keys than the table: the table is then merged
+    // and cleared before the next round.
+    constexpr int kRegionsPerWave = 4;
     const int lane = tid & 63, wave = tid >> 6, nwaves = bd >> 6;
-    for (int r0 = 0; r0 < a.W; r0 += nwaves) {
+    for (int r0 = 0; r0 < a.W; r0 += nwaves * kRegionsPerWave) {
         __syncthreads();
         // read into scalar registers: a barrier-containing branch on an LDS value must be
         // visibly uniform, or the compiler may split the wave around it
@@ -798,18 +800,30 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
             misc[3] = 0;
         }
         __syncthreads();
-        const int w = r0 + wave;
-        if (w >= a.W) continue;
-        const int64_t region = ((int64_t)w * a.P + part) * (int64_t)a.cap;
-        const int32_t cnt = (int32_t)a.r_count[(int64_t)w * a.P + part];
-        if (lane == 0) atomicAdd(&misc[3], (uint32_t)cnt);
+        // region g of this wave starts at row pre[g] of the run
+        int64_t rbase[kRegionsPerWave];
+        int32_t pre[kRegionsPerWave + 1];
+        pre[0] = 0;
+#pragma unroll
+        for (int g = 0; g < kRegionsPerWave; g++) {
+            const int w = r0 + wave * kRegionsPerWave + g;
+            const int32_t c = w < a.W ? (int32_t)a.r_count[(int64_t)w * a.P + part] : 0;
+            rbase[g] = ((int64_t)w * a.P + part) * (int64_t)a.cap;
+            pre[g + 1] = pre[g] + c;
+        }
+        const int32_t cnt = pre[kRegionsPerWave];
+        if (lane == 0 && cnt) atomicAdd(&misc[3], (uint32_t)cnt);
         for (int32_t j0 = 0; j0 < cnt; j0 += 64 * kP2Rows) {
             uint64_t key[kP2Rows];
             typename std::conditional<HAS_V, VT, int>::type val[kP2Rows];
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 const int32_t j = min(j0 + i * 64 + lane, cnt - 1);
-                get_rec<VT>(a.r_rec, region + j, key[i], val[i]);
+                int64_t at = rbase[0] + j;
+#pragma unroll
+                for (int g = 1; g < kRegionsPerWave; g++)
+                    if (j >= pre[g]) at = rbase[g] + (j - pre[g]);
+                get_rec<VT>(a.r_rec, at, key[i], val[i]);
             }
             // first two probes of every row at once; most rows find their key there
             uint32_t home[kP2Rows];
