@@ -607,8 +607,12 @@ __host__ __device__ inline size_t p2_lds_bytes(int32_t H2, int32_t W, uint32_t f
     return b;
 }
 
-__device__ __forceinline__ uint32_t p2_home(uint64_t h, int32_t H2) {
-    return (uint32_t)((((h >> 8) & 0xffffffffull) * (uint64_t)(uint32_t)H2) >> 32);
+// Home slot of a key in a partition's LDS table.  The partition already fixes the top bits of
+// the key's 64-bit hash, so the table uses the folded multiplicative hash instead (as the
+// phase-1 buckets do); the 64-bit hash is computed once per key, when it is merged.
+__device__ __forceinline__ uint32_t p2_home(uint64_t k, int32_t H2) {
+    const uint32_t f = ((uint32_t)k ^ (uint32_t)(k >> 32)) * 0x9e3779b1u;
+    return __umulhi(f, (uint32_t)H2);
 }
 
 template <typename VT, int FP>
@@ -674,7 +678,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         }
         return -1;
     };
-    auto insert = [&](uint64_t key) -> int { return insert_from(key, p2_home(hash64(key), H)); };
+    auto insert = [&](uint64_t key) -> int { return insert_from(key, p2_home(key, H)); };
     // rows of keys that found no LDS slot
     auto add_global = [&](uint64_t key, const Part& pr) {
         bool is_new = false;
@@ -829,7 +833,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
             uint32_t home[kP2Rows];
             uint64_t first[kP2Rows], second[kP2Rows];
 #pragma unroll
-            for (int i = 0; i < kP2Rows; i++) home[i] = p2_home(hash64(key[i]), H);
+            for (int i = 0; i < kP2Rows; i++) home[i] = p2_home(key[i], H);
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 first[i] = tkey[home[i]];
