@@ -93,14 +93,13 @@ __device__ __forceinline__ double dec_f64(uint64_t u) {
     return __longlong_as_double((long long)b);
 }
 
-// 64-bit finaliser (MurmurHash3 fmix64).
+// 64-bit key hash: a multiply by 2^64 / golden ratio (Fibonacci hashing: the high bits
+// depend on every bit of the key), then the high half folded into the low half, so the low
+// bits do too.  Tables take their sub-table from the high bits and the probe start from the
+// low bits.
 __device__ __host__ __forceinline__ uint64_t hash64(uint64_t k) {
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    k *= 0xc4ceb9fe1a85ec53ull;
-    k ^= k >> 33;
-    return k;
+    k *= 0x9e3779b97f4a7c15ull;
+    return k ^ (k >> 32);
 }
 
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
