@@ -53,8 +53,10 @@ template <typename VT, int MODE, typename VV>
 __device__ __forceinline__ bool row_passes(const PredSet& preds, uint32_t vpred, const int8_t* mask,
                                            int64_t r, const VV& v, bool vn, int64_t cv = 0, bool cn = false) {
     if (MODE == 1) return mask[r] != 0;
-    if constexpr (MODE == 3 && VTraits<VT>::has) return !vn && in_range<VT>(preds.vr, v);
-    if (MODE == 4) return !cn && col_in_range(preds, cv);
+    // & rather than &&: both sides are cheap and side-effect free, and a short circuit would
+    // become a divergent branch per row
+    if constexpr (MODE == 3 && VTraits<VT>::has) return !vn & in_range<VT>(preds.vr, v);
+    if (MODE == 4) return !cn & col_in_range(preds, cv);
     if (MODE == 2) {
         for (int p = 0; p < preds.n; p++) {
             const qk_predicate& q = preds.p[p];
@@ -484,8 +486,8 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             }
             o.key[k] = kk;
             o.val[k] = t.val[k];
-            const bool pass = r < n && row_passes<VT, MODE>(preds, vpred, mask, r, t.val[k], vn, t.cv[k],
-                                                            (t.cword[k] >> lane) & 1);
+            const bool pass = (r < n) & row_passes<VT, MODE>(preds, vpred, mask, r, t.val[k], vn, t.cv[k],
+                                                             (t.cword[k] >> lane) & 1);
             o.pass |= (uint32_t)pass << k;
             o.vnull |= (uint32_t)vn << k;
             o.knull |= (uint32_t)kn << k;
@@ -497,13 +499,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             if (!((o.pass >> k) & 1)) continue;
             const bool vn = (o.vnull >> k) & 1, kn = (o.knull >> k) & 1;
             const uint64_t kk = o.key[k];
-            int ls;
-            if (kn)
-                ls = H;
-            else if (kk == kEmpty)
-                ls = H + 1;
-            else
-                ls = heavy_find(lkey, nb, kk);
+            // the bucket is read for every row (a NULL key reads key 0's); the NULL key and the
+            // key equal to the empty marker then take their side slots by selection
+            int ls = heavy_find(lkey, nb, kk);
+            ls = kk == kEmpty ? H + 1 : ls;
+            ls = kn ? H : ls;
             Part pr;
             if constexpr (HAS_V) pr = row_part<VT>(o.val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
             if (ls >= 0) {
