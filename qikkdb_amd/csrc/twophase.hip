@@ -849,10 +849,10 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                     pr = row_part<NoValue>(NoValue{}, false);
                 }
                 const uint32_t next = home[i] + 1 == (uint32_t)H ? 0u : home[i] + 1;
-                int s;
-                if (first[i] == key[i]) s = (int)home[i];
-                else if (second[i] == key[i] && first[i] != kEmpty) s = (int)next;
-                else s = insert_from(key[i], home[i]);
+                // the two inline probes by selection; only keys further along branch
+                int s = ((second[i] == key[i]) & (first[i] != kEmpty)) ? (int)next : -1;
+                s = first[i] == key[i] ? (int)home[i] : s;
+                if (s < 0) s = insert_from(key[i], home[i]);
                 if (s >= 0)
                     add(s, 0, 1, pr.sum, pr.mn, pr.mx);
                 else
