@@ -14,7 +14,7 @@ namespace qk {
 namespace {
 
 constexpr int kP1Steps = 4;   // rows per thread per tile in phase 1
-constexpr int kP2Rows = 8;    // cold rows per thread per pass in phase 2
+constexpr int kP2Rows = 6;    // cold rows per thread per pass in phase 2
 constexpr int kBlock = 1024;
 constexpr int kBucket = 4;    // slots per bucket of the phase-1 heavy table
 constexpr int kSampleLds = 2048;
@@ -678,7 +678,10 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         }
         return -1;
     };
-    auto insert = [&](uint64_t key) -> int { return insert_from(key, p2_home(key, H)); };
+    // homes lie in the first H - kWindow slots, so the kWindow slots from a home never wrap
+    constexpr int kWindow = 4;
+    const int Hh = H - 8;
+    auto insert = [&](uint64_t key) -> int { return insert_from(key, p2_home(key, Hh)); };
     // rows of keys that found no LDS slot
     auto add_global = [&](uint64_t key, const Part& pr) {
         bool is_new = false;
@@ -829,15 +832,18 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                     if (j >= pre[g]) at = rbase[g] + (j - pre[g]);
                 get_rec<VT>(a.r_rec, at, key[i], val[i]);
             }
-            // first two probes of every row at once; most rows find their key there
+            // the kWindow slots from the home of every row at once: a key present in the table
+            // sits there unless its chain is longer (no deletions, so any match is the key's)
             uint32_t home[kP2Rows];
-            uint64_t first[kP2Rows], second[kP2Rows];
+            int hit[kP2Rows];
 #pragma unroll
-            for (int i = 0; i < kP2Rows; i++) home[i] = p2_home(key[i], H);
+            for (int i = 0; i < kP2Rows; i++) home[i] = p2_home(key[i], Hh);
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
-                first[i] = tkey[home[i]];
-                second[i] = tkey[home[i] + 1 == (uint32_t)H ? 0u : home[i] + 1];
+                int h = -1;
+#pragma unroll
+                for (int w = kWindow - 1; w >= 0; w--) h = tkey[home[i] + w] == key[i] ? (int)home[i] + w : h;
+                hit[i] = h;
             }
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
@@ -848,10 +854,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 } else {
                     pr = row_part<NoValue>(NoValue{}, false);
                 }
-                const uint32_t next = home[i] + 1 == (uint32_t)H ? 0u : home[i] + 1;
-                // the two inline probes by selection; only keys further along branch
-                int s = ((second[i] == key[i]) & (first[i] != kEmpty)) ? (int)next : -1;
-                s = first[i] == key[i] ? (int)home[i] : s;
+                int s = hit[i];  // only keys further along, or new, branch
                 if (s < 0) s = insert_from(key[i], home[i]);
                 if (s >= 0)
                     add(s, 0, 1, pr.sum, pr.mn, pr.mx);
