@@ -239,6 +239,24 @@ __device__ __forceinline__ int64_t g_slot1(const GView& g, uint64_t key, bool kn
     return -1;
 }
 
+// As g_slot1 for a key that is usually absent (the merge of a workgroup's table into its own
+// sub-table): every probe is one CAS instead of a read and then a CAS.
+__device__ __forceinline__ int64_t g_slot1_claim(const GView& g, uint64_t key, uint64_t h, bool* is_new) {
+    *is_new = false;
+    uint64_t s = home_of(g, h);
+    for (uint64_t p = 0; p <= g.sub_mask; p++) {
+        const unsigned long long old = atomicCAS((unsigned long long*)(g.keys + s), (unsigned long long)kEmpty,
+                                                 (unsigned long long)key);
+        if (old == kEmpty) {
+            *is_new = true;
+            return (int64_t)s;
+        }
+        if (old == key) return (int64_t)s;
+        s = next_in_sub(g, s);
+    }
+    return -1;
+}
+
 // Slot of a multi-word key: the last word is claimed (0 -> kBusy), the other words written,
 // then the last word published with release order; readers that see kBusy wait for it.
 template <int KW>
@@ -310,6 +328,16 @@ __device__ __forceinline__ void g_accum_owned(const GView& g, int64_t s, const P
         if ((fields & F_MIN) && p.mn < g.mn[s]) g.mn[s] = p.mn;
         if ((fields & F_MAX) && p.mx > g.mx[s]) g.mx[s] = p.mx;
     }
+}
+
+// The states of a slot this workgroup has just claimed (still at their initial values, and no
+// other workgroup writes its sub-table): stored without reading them first.
+__device__ __forceinline__ void g_store_new(const GView& g, int64_t s, const Part& p, uint32_t fields) {
+    g.cnt[s] = p.cnt;
+    if (fields & F_CNTALL) g.cntall[s] = p.cntall;
+    if (fields & F_SUM) g.sum[s] = p.cnt ? p.sum : 0ull;
+    if (fields & F_MIN) g.mn[s] = p.cnt ? p.mn : ~0ull;
+    if (fields & F_MAX) g.mx[s] = p.cnt ? p.mx : 0ull;
 }
 
 // Count `n` new keys of one workgroup; reports a full table against the logical capacity.

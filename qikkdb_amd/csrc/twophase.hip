@@ -724,9 +724,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
             const uint64_t key = tkey[s];
             if (key == kEmpty) continue;
             bool is_new = false;
-            const int64_t gs = g_slot1(g, key, false, hash64(key), &is_new);
+            const int64_t gs = g_slot1_claim(g, key, hash64(key), &is_new);
             if (is_new) atomicAdd(&misc[1], 1u);
-            if (gs >= 0) g_accum_owned(g, gs, part_of_slot(s), fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+            if (gs < 0) g_set_error(g, QK_HASH_TABLE_FULL);
+            else if (is_new) g_store_new(g, gs, part_of_slot(s), fields);
+            else g_accum_owned(g, gs, part_of_slot(s), fields);
             clear_slot(s);
         }
         __syncthreads();
