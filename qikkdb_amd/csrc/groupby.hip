@@ -820,10 +820,19 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     uint64_t cap = 64;
     while (cap < 2 * logical) cap <<= 1;
     h->cap = cap;
-    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only)
+    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only); up to
+    // 1024 of them when that lets a sub-table fit the LDS, so that the two-phase path folds
+    // straight into its sub-tables
     int32_t parts = 1;
-    if (ks.kw == 1)
+    if (ks.kw == 1) {
         while (parts < 256 && cap / (uint64_t)(parts * 2) >= 1024) parts *= 2;
+        const size_t budget = std::min<size_t>(device_info().lds_per_block, 160 * 1024);
+        int32_t more = parts;
+        while (more < 1024 && p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) > budget &&
+               cap / (uint64_t)(more * 2) >= 1024)
+            more *= 2;
+        if (p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) <= budget) parts = more;
+    }
     h->parts = parts;
     const size_t slots = (size_t)(cap + 2);
     int st = QK_SUCCESS;

@@ -878,6 +878,224 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
     if (tid == 0) g_count_new(g, misc[1]);
 }
 
+// ---- phase 2, owned sub-table -------------------------------------------------------------
+
+// Phase 2 when a partition's sub-table fits the LDS.  The workgroup owns its sub-table: it
+// loads the sub-table's keys, folds the heavy partials and the partition's cold rows into
+// deltas at the global table's own slot positions (the same home slot and probe order as
+// g_slot1), and adds the deltas to the sub-table's states.  Every global access is coalesced
+// and no row needs a global probe or atomic.  Counts are 32-bit deltas (one flush holds at
+// most 2^30 rows); the global states stay 64-bit.  The kernel has no static LDS, so the
+// sub-table may use all of it.
+template <typename VT, int FP>
+__global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fields, P2Args a) {
+    extern __shared__ __align__(16) uint64_t lds[];
+    constexpr bool HAS_V = VTraits<VT>::has;
+    constexpr bool REAL = VTraits<VT>::real;
+    const bool f_sum = FP == kFieldsSum ? true : (fields & F_SUM) != 0;
+    const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
+    const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
+    const bool f_all = (fields & F_CNTALL) != 0;
+    const int tid = threadIdx.x, bd = blockDim.x, part = blockIdx.x;
+    const uint32_t S = (uint32_t)g.sub_mask + 1;
+    const int64_t base = (int64_t)part << g.sub_bits;
+    // 8-byte arrays first, then the 4-byte ones
+    uint64_t* okey = lds;
+    unsigned long long* p = (unsigned long long*)(okey + S);
+    unsigned long long* osum = f_sum ? p : nullptr;
+    p += f_sum ? S : 0;
+    unsigned long long* omin = f_min ? p : nullptr;
+    p += f_min ? S : 0;
+    unsigned long long* omax = f_max ? p : nullptr;
+    p += f_max ? S : 0;
+    uint32_t* ocnt = (uint32_t*)p;
+    uint32_t* oall = f_all ? ocnt + S : nullptr;
+
+    for (uint32_t s = tid; s < S; s += bd) {
+        okey[s] = g.keys[base + s];
+        ocnt[s] = 0;
+        if (oall) oall[s] = 0;
+        if (osum) osum[s] = 0;
+        if (omin) omin[s] = ~0ull;
+        if (omax) omax[s] = 0;
+    }
+    __syncthreads();
+
+    uint32_t nnew = 0;  // keys this thread added to the table
+    // slot of a key (claimed when absent), probing from h & sub_mask as g_slot1 does
+    auto claim_from = [&](uint64_t key, uint32_t s) -> int {
+        for (uint32_t it = 0; it < S; it++) {
+            const uint64_t cur = okey[s];
+            if (cur == key) return (int)s;
+            if (cur == kEmpty) {
+                const unsigned long long old = atomicCAS((unsigned long long*)&okey[s], (unsigned long long)kEmpty,
+                                                         (unsigned long long)key);
+                if (old == kEmpty) {
+                    nnew++;
+                    return (int)s;
+                }
+                if (old == key) return (int)s;
+            }
+            s = (s + 1) & (S - 1);
+        }
+        return -1;
+    };
+    auto add = [&](int s, uint32_t rows, uint32_t cnt, uint64_t sum, uint64_t mn, uint64_t mx) {
+        if (oall && rows) atomicAdd(&oall[s], rows);
+        if (cnt) {
+            atomicAdd(&ocnt[s], cnt);
+            if (osum) {
+                if (REAL)
+                    atomicAdd((double*)&osum[s], __longlong_as_double((long long)sum));
+                else
+                    atomicAdd(&osum[s], (unsigned long long)sum);
+            }
+            if (omin) atomicMin(&omin[s], (unsigned long long)mn);
+            if (omax) atomicMax(&omax[s], (unsigned long long)mx);
+        }
+    };
+
+    // heavy partials of this partition's heavy keys, summed over the phase-1 workgroups (a
+    // thread owns one key and a strided subset of the workgroups)
+    const int32_t hk0 = a.meta[2 + part], hk1 = a.meta[2 + part + 1];
+    const int32_t nk = hk1 - hk0;
+    const int32_t stride_k = a.kmax + 2;
+    if (nk > 0) {
+        const int per = nk <= bd ? bd / nk : 1;
+        const int span = nk <= bd ? per * nk : bd;
+        if (tid < span) {
+            const int w0 = nk <= bd ? tid / nk : 0;
+            for (int ki = nk <= bd ? tid % nk : tid; ki < nk; ki += (nk <= bd ? nk : bd)) {
+                uint64_t rows = 0, cnt = 0, isum = 0, mn = ~0ull, mx = 0;
+                double dsum = 0.0;
+                for (int w = w0; w < a.W; w += per) {
+                    const int64_t at = (int64_t)w * stride_k + hk0 + ki;
+                    rows += a.p_rows[at];
+                    cnt += a.p_cnt[at];
+                    if (osum) {
+                        if (REAL) dsum += __longlong_as_double((long long)a.p_sum[at]);
+                        else isum += a.p_sum[at];
+                    }
+                    if (omin) mn = min(mn, (uint64_t)a.p_min[at]);
+                    if (omax) mx = max(mx, (uint64_t)a.p_max[at]);
+                }
+                if (!rows) continue;
+                const uint64_t key = a.heavy[hk0 + ki];
+                const int s = claim_from(key, (uint32_t)(hash64(key) & g.sub_mask));
+                if (s >= 0) add(s, (uint32_t)rows, (uint32_t)cnt, REAL ? (uint64_t)__double_as_longlong(dsum) : isum, mn, mx);
+                else g_set_error(g, QK_HASH_TABLE_FULL);
+            }
+        }
+    }
+
+    // cold rows: every wave walks the partition's regions of kRegionsPerWave phase-1
+    // workgroups at a time as one run of rows
+    constexpr int kRegionsPerWave = 4;
+    constexpr int kWindow = 4;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = bd >> 6;
+    for (int r0 = wave * kRegionsPerWave; r0 < a.W; r0 += nwaves * kRegionsPerWave) {
+        int64_t rbase[kRegionsPerWave];
+        int32_t pre[kRegionsPerWave + 1];
+        pre[0] = 0;
+#pragma unroll
+        for (int q = 0; q < kRegionsPerWave; q++) {
+            const int w = r0 + q;
+            const int32_t c = w < a.W ? (int32_t)a.r_count[(int64_t)w * a.P + part] : 0;
+            rbase[q] = ((int64_t)w * a.P + part) * (int64_t)a.cap;
+            pre[q + 1] = pre[q] + c;
+        }
+        const int32_t cnt = pre[kRegionsPerWave];
+        for (int32_t j0 = 0; j0 < cnt; j0 += 64 * kP2Rows) {
+            uint64_t key[kP2Rows];
+            typename std::conditional<HAS_V, VT, int>::type val[kP2Rows];
+#pragma unroll
+            for (int i = 0; i < kP2Rows; i++) {
+                const int32_t j = min(j0 + i * 64 + lane, cnt - 1);
+                int64_t at = rbase[0] + j;
+#pragma unroll
+                for (int q = 1; q < kRegionsPerWave; q++)
+                    if (j >= pre[q]) at = rbase[q] + (j - pre[q]);
+                get_rec<VT>(a.r_rec, at, key[i], val[i]);
+            }
+            // the kWindow slots from every row's home at once; a key present in the table sits
+            // there unless its chain is longer (no deletions, so any match is the key's)
+            uint32_t home[kP2Rows];
+            int hit[kP2Rows];
+#pragma unroll
+            for (int i = 0; i < kP2Rows; i++) home[i] = (uint32_t)(hash64(key[i]) & g.sub_mask);
+#pragma unroll
+            for (int i = 0; i < kP2Rows; i++) {
+                int h = -1;
+#pragma unroll
+                for (int w = kWindow - 1; w >= 0; w--) {
+                    const uint32_t sw = (home[i] + (uint32_t)w) & (S - 1);
+                    h = okey[sw] == key[i] ? (int)sw : h;
+                }
+                hit[i] = h;
+            }
+#pragma unroll
+            for (int i = 0; i < kP2Rows; i++) {
+                if (j0 + i * 64 + lane >= cnt) continue;
+                Part pr;
+                if constexpr (HAS_V) pr = row_part<VT>(val[i], false); else pr = row_part<NoValue>(NoValue{}, false);
+                int s = hit[i];
+                if (s < 0) s = claim_from(key[i], home[i]);
+                if (s >= 0) add(s, 1, 1, pr.sum, pr.mn, pr.mx);
+                else g_set_error(g, QK_HASH_TABLE_FULL);
+            }
+        }
+    }
+    __syncthreads();
+    // every key goes back (a claimed key may carry only NULL-value rows); states where a slot
+    // took rows
+    for (uint32_t s = tid; s < S; s += bd) {
+        g.keys[base + s] = okey[s];
+        const uint32_t c = ocnt[s], r = oall ? oall[s] : 0u;
+        if (r) g.cntall[base + s] += r;
+        if (!c) continue;
+        g.cnt[base + s] += c;
+        if (osum) {
+            if (REAL)
+                g.sum[base + s] = (unsigned long long)__double_as_longlong(
+                    __longlong_as_double((long long)g.sum[base + s]) + __longlong_as_double((long long)osum[s]));
+            else
+                g.sum[base + s] += osum[s];
+        }
+        if (omin && omin[s] < g.mn[base + s]) g.mn[base + s] = omin[s];
+        if (omax && omax[s] > g.mx[base + s]) g.mx[base + s] = omax[s];
+    }
+    // NULL key and key == kEmpty: side slots, summed over the phase-1 workgroups
+    if (part == 0 && tid < 2) {
+        Part q;
+        q.cnt = 0;
+        q.cntall = 0;
+        q.sum = 0;
+        q.mn = ~0ull;
+        q.mx = 0;
+        double dsum = 0.0;
+        for (int w = 0; w < a.W; w++) {
+            const int64_t at = (int64_t)w * stride_k + a.kmax + tid;
+            q.cntall += a.p_rows[at];
+            q.cnt += a.p_cnt[at];
+            if (osum) {
+                if (REAL) dsum += __longlong_as_double((long long)a.p_sum[at]);
+                else q.sum += a.p_sum[at];
+            }
+            if (omin) q.mn = min(q.mn, (uint64_t)a.p_min[at]);
+            if (omax) q.mx = max(q.mx, (uint64_t)a.p_max[at]);
+        }
+        if (REAL) q.sum = (uint64_t)__double_as_longlong(dsum);
+        if (q.cntall) {
+            const uint64_t gs = g.cap + tid;
+            if (g_claim_side(g, gs) && tid == 1) nnew++;
+            g_accum_owned(g, (int64_t)gs, q, fields);
+        }
+    }
+    // new keys: a sum per wavefront, counted once per wavefront
+    for (int d = 32; d >= 1; d >>= 1) nnew += __shfl_xor(nnew, d, 64);
+    if ((tid & 63) == 0) g_count_new(g, nnew);
+}
+
 // ---- launch helpers ----------------------------------------------------------------------
 
 int grid_cap(int64_t items, int per, int cap) {
@@ -995,10 +1213,17 @@ int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
     a.p_max = b.p_max;
     a.r_rec = b.r_rec;
     a.r_count = b.r_count;
-    if ((fields & F_SUM) && !(fields & (F_MIN | F_MAX)))
+    const bool sum_only = (fields & F_SUM) && !(fields & (F_MIN | F_MAX));
+    const size_t olds = p2o_lds_bytes((int64_t)g.sub_mask + 1, fields);
+    if (b.P == (int32_t)(g.cap >> g.sub_bits) && olds <= std::min<size_t>(device_info().lds_per_block, 160 * 1024)) {
+        // the sub-tables fit the LDS: each workgroup folds into its own sub-table directly
+        if (sum_only) k_tp_phase2_owned<VT, kFieldsSum><<<b.P, b.block, olds, s>>>(g, fields, a);
+        else k_tp_phase2_owned<VT, kFieldsAny><<<b.P, b.block, olds, s>>>(g, fields, a);
+    } else if (sum_only) {
         k_tp_phase2<VT, kFieldsSum><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
-    else
+    } else {
         k_tp_phase2<VT, kFieldsAny><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
+    }
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }

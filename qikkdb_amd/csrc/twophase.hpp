@@ -60,6 +60,17 @@ struct TwoPhaseBufs {
 
 constexpr int kHistBins = 1024;
 
+// Bytes per slot of a sub-table held in LDS by phase 2 (the key, 32-bit count deltas and
+// 64-bit state deltas), and the LDS bytes of a sub-table of S slots.  Phase 2 folds straight
+// into the global sub-tables when these fit the LDS.
+__host__ __device__ inline int owned_slot_bytes(uint32_t fields) {
+    return 8 + 4 + ((fields & F_CNTALL) ? 4 : 0) + ((fields & F_SUM) ? 8 : 0) + ((fields & F_MIN) ? 8 : 0) +
+           ((fields & F_MAX) ? 8 : 0);
+}
+__host__ __device__ inline size_t p2o_lds_bytes(int64_t S, uint32_t fields) {
+    return (size_t)S * (size_t)owned_slot_bytes(fields);
+}
+
 // Whether the two-phase path applies: single-word keys and a value type it handles.
 bool twophase_supported(int32_t kw, int32_t value_type);
 
