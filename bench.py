@@ -416,7 +416,7 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=4096, help="batched workload: queries per batch")
     ap.add_argument("--plain-queries", type=int, default=128,
                     help="batched workload: queries timed on the one-at-a-time path")
-    ap.add_argument("--rates", default="500,1000,2000,4000", help="serving workload: arrival rates (q/s)")
+    ap.add_argument("--rates", default="500,1000,1500,2000,3000,4000", help="serving workload: arrival rates (q/s)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving workload: seconds of arrivals per rate")
     ap.add_argument("--engines", type=int, default=2, help="serving workload: batch engines")
     ap.add_argument("--batch-tuning", default="",

@@ -751,6 +751,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
                 uint32_t rows = 0, cnt = 0;
                 uint64_t isum = 0, mn = ~0ull, mx = 0;
                 double dsum = 0.0;
+#pragma unroll 8
                 for (int w = w0; w < a.W; w += per) {
                     const int64_t at = (int64_t)w * stride_k + hk0 + ki;
                     rows += a.p_rows[at];
@@ -968,6 +969,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
             for (int ki = nk <= bd ? tid % nk : tid; ki < nk; ki += (nk <= bd ? nk : bd)) {
                 uint64_t rows = 0, cnt = 0, isum = 0, mn = ~0ull, mx = 0;
                 double dsum = 0.0;
+#pragma unroll 8
                 for (int w = w0; w < a.W; w += per) {
                     const int64_t at = (int64_t)w * stride_k + hk0 + ki;
                     rows += a.p_rows[at];
@@ -1064,31 +1066,42 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
         if (omin && omin[s] < g.mn[base + s]) g.mn[base + s] = omin[s];
         if (omax && omax[s] > g.mx[base + s]) g.mx[base + s] = omax[s];
     }
-    // NULL key and key == kEmpty: side slots, summed over the phase-1 workgroups
-    if (part == 0 && tid < 2) {
-        Part q;
-        q.cnt = 0;
-        q.cntall = 0;
-        q.sum = 0;
-        q.mn = ~0ull;
-        q.mx = 0;
-        double dsum = 0.0;
-        for (int w = 0; w < a.W; w++) {
-            const int64_t at = (int64_t)w * stride_k + a.kmax + tid;
-            q.cntall += a.p_rows[at];
-            q.cnt += a.p_cnt[at];
-            if (osum) {
-                if (REAL) dsum += __longlong_as_double((long long)a.p_sum[at]);
-                else q.sum += a.p_sum[at];
+    // NULL key and key == kEmpty: side slots, summed over the phase-1 workgroups (a strided
+    // share per thread, then a sum per wavefront added with device atomics)
+    if (part == 0) {
+        for (int side = 0; side < 2; side++) {
+            uint64_t rows = 0, cnt = 0, isum = 0, mn = ~0ull, mx = 0;
+            double dsum = 0.0;
+            for (int w = tid; w < a.W; w += bd) {
+                const int64_t at = (int64_t)w * stride_k + a.kmax + side;
+                rows += a.p_rows[at];
+                cnt += a.p_cnt[at];
+                if (osum) {
+                    if (REAL) dsum += __longlong_as_double((long long)a.p_sum[at]);
+                    else isum += a.p_sum[at];
+                }
+                if (omin) mn = min(mn, (uint64_t)a.p_min[at]);
+                if (omax) mx = max(mx, (uint64_t)a.p_max[at]);
             }
-            if (omin) q.mn = min(q.mn, (uint64_t)a.p_min[at]);
-            if (omax) q.mx = max(q.mx, (uint64_t)a.p_max[at]);
-        }
-        if (REAL) q.sum = (uint64_t)__double_as_longlong(dsum);
-        if (q.cntall) {
-            const uint64_t gs = g.cap + tid;
-            if (g_claim_side(g, gs) && tid == 1) nnew++;
-            g_accum_owned(g, (int64_t)gs, q, fields);
+            for (int d = 32; d >= 1; d >>= 1) {
+                rows += __shfl_xor(rows, d, 64);
+                cnt += __shfl_xor(cnt, d, 64);
+                isum += __shfl_xor(isum, d, 64);
+                dsum += __shfl_xor(dsum, d, 64);
+                mn = min(mn, (uint64_t)__shfl_xor(mn, d, 64));
+                mx = max(mx, (uint64_t)__shfl_xor(mx, d, 64));
+            }
+            if ((tid & 63) == 0 && rows) {
+                const uint64_t gs = g.cap + side;
+                if (g_claim_side(g, gs) && side == 1) nnew++;
+                Part q;
+                q.cntall = rows;
+                q.cnt = cnt;
+                q.sum = REAL ? (uint64_t)__double_as_longlong(dsum) : isum;
+                q.mn = mn;
+                q.mx = mx;
+                g_accum_atomic(g, (int64_t)gs, q, fields);
+            }
         }
     }
     // new keys: a sum per wavefront, counted once per wavefront
