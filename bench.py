@@ -330,7 +330,7 @@ def run_serving(args) -> int:
     rng = np.random.default_rng(11)
     rates = [float(x) for x in args.rates.split(",")]
     rows = []
-    with S.Server(*ptrs, n, blk, 4096, S.Config(engines=args.engines, tuning=tuned)) as srv:
+    with S.Server(*ptrs, n, blk, 4096, S.Config(engines=args.engines, tuning=tuned, wait_ms=args.wait_ms)) as srv:
         srv.replay(tune_qs, np.zeros(len(tune_qs)))  # warm-up: buffers and pools
         for lam in rates:
             m = int(lam * args.serve_seconds)
@@ -381,7 +381,8 @@ def run_serving(args) -> int:
         "config": {"workload": "serving: open-loop Poisson arrivals drawn from the batched query mix over "
                                "the device-resident 512M-row table; continuous batching, admission by "
                                "projected table bytes under a 200 GB cap",
-                   "rows": n, "block_rows": blk, "engines": args.engines, "seconds_per_rate": args.serve_seconds,
+                   "rows": n, "block_rows": blk, "engines": args.engines, "wait_ms": args.wait_ms,
+                   "seconds_per_rate": args.serve_seconds,
                    "tuning": tuned.__dict__},
         "serving": rows,
         "roofline": None,
@@ -419,6 +420,8 @@ def main() -> int:
     ap.add_argument("--rates", default="500,1000,1500,2000,3000,4000", help="serving workload: arrival rates (q/s)")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="serving workload: seconds of arrivals per rate")
     ap.add_argument("--engines", type=int, default=2, help="serving workload: batch engines")
+    ap.add_argument("--wait-ms", type=float, default=0.0,
+                    help="serving workload: wait of a second batch (0 = half the last batch's time)")
     ap.add_argument("--batch-tuning", default="",
                     help="batched workload: 'tile_rows,lds_groups,wgs_per_cu' instead of autotuning")
     ap.add_argument("--rows", type=int, default=0, help="rows (default: the workload's size)")
