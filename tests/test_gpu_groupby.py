@@ -311,6 +311,28 @@ def test_two_phase_uniform_high_cardinality_and_sentinel():
     compare(gpu, cpu, 4)
 
 
+@pytest.mark.parametrize("flags", [0, qk.GB_ASYNC])
+@pytest.mark.parametrize("sum_only", [False, True])
+def test_two_phase_merge_path_for_large_tables(flags, sum_only):
+    """A table of 2^24 slots has sub-tables too large for the LDS even at 1024 of them, so
+    phase 2 aggregates in an LDS hash table and merges into its own sub-table (the path of
+    the 16.7M-group composite workload); heavy hitters, a long tail and NULL keys."""
+    rng = np.random.default_rng(91)
+    n = 2_000_000
+    keys = np.where(rng.random(n) < 0.5, datagen.zipf_keys(13, 1.1, 5000, n),
+                    rng.integers(-(2**40), 2**40, n)).astype(np.int64)
+    kn = [rng.random(n) < 0.001]
+    if sum_only:
+        vals = rng.lognormal(0, 1, n).astype(np.float32)
+        aggs, real = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT], (0, 1)
+    else:
+        vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+        aggs, real = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT], ()
+    gpu = gpu_group_by(aggs, [keys], vals, kn, None, block=1 << 19, capacity=5_000_000, path=1, flags=flags)
+    cpu = cpu_group_by(aggs, [keys], vals, kn, None)
+    compare(gpu, cpu, len(aggs), real_aggs=real)
+
+
 def test_two_phase_small_table_all_heavy_over_blocks():
     d = datagen.small(1 << 21, seed=4)
     aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
