@@ -201,26 +201,6 @@ __global__ void __launch_bounds__(256) k_batch_sample_ts(const int64_t* __restri
     }
 }
 
-template <typename F>
-__device__ __forceinline__ void for_passing_rows(const BArgs& p, const BQuery& Q, int64_t r_begin, int64_t r_end,
-                                                 F f) {
-    const int tid = threadIdx.x, bd = blockDim.x;
-    for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
-        int64_t tsv[kBRows];
-#pragma unroll
-        for (int k = 0; k < kBRows; k++) {
-            const int64_t r = min(base + (int64_t)k * bd + tid, r_end - 1);
-            tsv[k] = p.ts[r];
-        }
-#pragma unroll
-        for (int k = 0; k < kBRows; k++) {
-            const int64_t r = base + (int64_t)k * bd + tid;
-            if (r >= r_end || tsv[k] < Q.ts_lo || tsv[k] >= Q.ts_hi) continue;
-            f(r, key_of(p.a[r], p.b[r], p.b_range, Q));
-        }
-    }
-}
-
 // Claims the next pass-1 unit of a wave: first from the list of this workgroup's die (units
 // of one tile share that die's L2), then from the other lists once it is empty.  -1 when
 // every list is empty.  `xi` is the caller's position in that order.
@@ -280,14 +260,32 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
                 lcnt[g] = 0;
             }
             __syncthreads();
-            for_passing_rows(p, Q, r_begin, r_end, [&](int64_t r, uint32_t key) {
-                const int64_t val = p.v[r];
-                const uint64_t e = enc_i64(val);
-                atomicAdd(&lcnt[key], 1u);
-                atomicAdd((unsigned long long*)&lsum[key], (unsigned long long)val);
-                atomicMin((unsigned long long*)&lmn[key], (unsigned long long)e);
-                atomicMax((unsigned long long*)&lmx[key], (unsigned long long)e);
-            });
+            // every column of a step is loaded at once (one wait per step); most of these
+            // lines are read by other queries of the same tile anyway
+            for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
+                int64_t tsv[kBRows], vv[kBRows];
+                int32_t av[kBRows], bv[kBRows];
+#pragma unroll
+                for (int k = 0; k < kBRows; k++) {
+                    const int64_t r = min(base + (int64_t)k * bd + tid, r_end - 1);
+                    tsv[k] = p.ts[r];
+                    av[k] = p.a[r];
+                    bv[k] = p.b[r];
+                    vv[k] = p.v[r];
+                }
+#pragma unroll
+                for (int k = 0; k < kBRows; k++) {
+                    const int64_t r = base + (int64_t)k * bd + tid;
+                    const bool pass = (r < r_end) & (tsv[k] >= Q.ts_lo) & (tsv[k] < Q.ts_hi);
+                    if (!pass) continue;
+                    const uint32_t key = key_of(av[k], bv[k], p.b_range, Q);
+                    const uint64_t e = enc_i64(vv[k]);
+                    atomicAdd(&lcnt[key], 1u);
+                    atomicAdd((unsigned long long*)&lsum[key], (unsigned long long)vv[k]);
+                    atomicMin((unsigned long long*)&lmn[key], (unsigned long long)e);
+                    atomicMax((unsigned long long*)&lmx[key], (unsigned long long)e);
+                }
+            }
             __syncthreads();
             for (int g = tid; g < Q.mod; g += bd)
                 if (lcnt[g]) add_global(p, Q.out + g, lcnt[g], lsum[g], lmn[g], lmx[g]);
@@ -305,15 +303,21 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
             int word = wid;
             for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
                 int64_t tsv[kBRows];
+                int32_t av[kBRows], bv[kBRows];
 #pragma unroll
-                for (int j = 0; j < kBRows; j++) tsv[j] = p.ts[min(base + (int64_t)j * bd + tid, r_end - 1)];
+                for (int j = 0; j < kBRows; j++) {
+                    const int64_t r = min(base + (int64_t)j * bd + tid, r_end - 1);
+                    tsv[j] = p.ts[r];
+                    av[j] = p.a[r];
+                    bv[j] = p.b[r];
+                }
 #pragma unroll
                 for (int j = 0; j < kBRows; j++, word += nwv) {
                     const int64_t r = base + (int64_t)j * bd + tid;
-                    const bool pass = r < r_end && tsv[j] >= Q.ts_lo && tsv[j] < Q.ts_hi;
+                    const bool pass = (r < r_end) & (tsv[j] >= Q.ts_lo) & (tsv[j] < Q.ts_hi);
                     const uint64_t m = __ballot(pass);
                     if (lane == 0) mask[word] = m;
-                    if (pass) atomicAdd(&hist[key_of(p.a[r], p.b[r], p.b_range, Q) >> cbits], 1u);
+                    if (pass) atomicAdd(&hist[key_of(av[j], bv[j], p.b_range, Q) >> cbits], 1u);
                 }
             }
             __syncthreads();
@@ -341,12 +345,20 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass1(BArgs p, int32_t wave
             const int64_t sbase = uniform64(s_base);
             word = wid;
             for (int64_t base = r_begin; base < r_end; base += (int64_t)bd * kBRows) {
+                int32_t av[kBRows], bv[kBRows];
+                int64_t vv[kBRows];
+#pragma unroll
+                for (int j = 0; j < kBRows; j++) {
+                    const int64_t r = min(base + (int64_t)j * bd + tid, r_end - 1);
+                    av[j] = p.a[r];
+                    bv[j] = p.b[r];
+                    vv[j] = p.v[r];
+                }
 #pragma unroll
                 for (int j = 0; j < kBRows; j++, word += nwv) {
                     if (!((mask[word] >> lane) & 1)) continue;
-                    const int64_t r = base + (int64_t)j * bd + tid;
-                    const uint32_t key = key_of(p.a[r], p.b[r], p.b_range, Q);
-                    const int64_t val = p.v[r];
+                    const uint32_t key = key_of(av[j], bv[j], p.b_range, Q);
+                    const int64_t val = vv[j];
                     if (sbase >= 0) {
                         const uint32_t at = atomicAdd(&cur[key >> cbits], 1u);
                         p.sk[sbase + at] = key;
