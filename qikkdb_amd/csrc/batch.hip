@@ -423,14 +423,26 @@ __global__ void __launch_bounds__(kBThreads) k_batch_pass2(BArgs p, int32_t wave
             if (base < 0) continue;  // that unit's rows took the atomic path
             const int32_t* d = p.desc + Q.desc + (int64_t)k * (nch + 1);
             const int32_t lo = d[j], hi = d[j + 1];
-            for (int32_t i = lo + lane; i < hi; i += 64) {
-                const uint32_t g = p.sk[base + i] - (uint32_t)key0;
-                const int64_t val = p.sv[base + i];
-                const uint64_t e = enc_i64(val);
-                atomicAdd(&lcnt[g], 1u);
-                atomicAdd((unsigned long long*)&lsum[g], (unsigned long long)val);
-                atomicMin((unsigned long long*)&lmn[g], (unsigned long long)e);
-                atomicMax((unsigned long long*)&lmx[g], (unsigned long long)e);
+            constexpr int kU = 4;  // rows per lane per iteration, loaded before any is added
+            for (int32_t i0 = lo; i0 < hi; i0 += 64 * kU) {
+                uint32_t kk[kU];
+                int64_t vv[kU];
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    const int32_t i = min(i0 + u * 64 + lane, hi - 1);
+                    kk[u] = p.sk[base + i];
+                    vv[u] = p.sv[base + i];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; u++) {
+                    if (i0 + u * 64 + lane >= hi) continue;
+                    const uint32_t g = kk[u] - (uint32_t)key0;
+                    const uint64_t e = enc_i64(vv[u]);
+                    atomicAdd(&lcnt[g], 1u);
+                    atomicAdd((unsigned long long*)&lsum[g], (unsigned long long)vv[u]);
+                    atomicMin((unsigned long long*)&lmn[g], (unsigned long long)e);
+                    atomicMax((unsigned long long*)&lmx[g], (unsigned long long)e);
+                }
             }
         }
         __syncthreads();
