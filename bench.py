@@ -346,24 +346,30 @@ def run_serving(args) -> int:
             svc = rng.choice(service_ms, m)
             plain = fifo_latencies(arrival, svc)
             plain_qps = m / ((plain + arrival).max() / 1e3)  # requests over the time the last one is done
-            # the highest rate the plain path sustains with its p99 within the served p99
+            # the highest rate the plain path sustains with its p99 within the served p99; none
+            # when even its single-query service time exceeds that p99 at the 99th percentile
+            reachable = np.percentile(service_ms, 99) <= p99
             lo, hi = 1.0, 1e3 / service_ms.mean()
-            for _ in range(30):
+            for _ in range(30 if reachable else 0):
                 mid = 0.5 * (lo + hi)
                 arr2 = np.cumsum(rng.exponential(1000.0 / mid, max(64, int(mid * args.serve_seconds))))
                 ok = np.percentile(fifo_latencies(arr2, rng.choice(service_ms, len(arr2))), 99) <= p99
                 lo, hi = (mid, hi) if ok else (lo, mid)
+            if not reachable:
+                lo = 0.0
             rows.append({"offered_qps": lam, "requests": m, "served_qps": qps, "p50_ms": p50, "p99_ms": p99,
                          "batches": after["batches"] - before["batches"],
                          "peak_admitted_gb": after["peak_admitted_bytes"] / 1e9,
                          "plain_same_rate": {"served_qps": plain_qps,
                                              "p50_ms": float(np.percentile(plain, 50)),
                                              "p99_ms": float(np.percentile(plain, 99))},
-                         "plain_qps_at_equal_p99": lo, "ratio_at_equal_p99": qps / lo})
+                         "plain_qps_at_equal_p99": lo, "ratio_at_equal_p99": qps / lo if lo > 0 else None})
+            cmp = (f"plain sustains {lo:.0f} q/s within p99 {p99:.0f} ms -> {qps / lo:.1f}x" if lo > 0 else
+                   f"plain reaches no p99 of {p99:.0f} ms at any rate (its own service p99 is "
+                   f"{np.percentile(service_ms, 99):.0f} ms)")
             log(f"rate {lam:.0f} q/s: {m} requests served at {qps:.0f} q/s in {after['batches'] - before['batches']} "
                 f"batches, latency p50 {p50:.0f} ms p99 {p99:.0f} ms; plain at this rate p50 "
-                f"{np.percentile(plain, 50):.0f} ms p99 {np.percentile(plain, 99):.0f} ms; plain sustains "
-                f"{lo:.0f} q/s within p99 {p99:.0f} ms -> {qps / lo:.1f}x")
+                f"{np.percentile(plain, 50):.0f} ms p99 {np.percentile(plain, 99):.0f} ms; {cmp}")
     best = max(rows, key=lambda r: r["served_qps"])
     out = {
         "metric": METRIC,
