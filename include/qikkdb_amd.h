@@ -296,7 +296,8 @@ int qk_gb_set_defer_rows(qk_groupby h, int64_t rows);
  * out[0] path of the last block (0 one-pass, 1 two-phase), out[1] heavy keys, out[2] heavy
  * count threshold, out[3] rows written to partition regions by the last block, out[4]
  * phase-1 workgroups, out[5] partitions, out[6] rows per region, out[7] phase-1 LDS slots,
- * out[8] phase-2 LDS slots, out[9] heavy key limit.  n = entries to fill (at most 10). */
+ * out[8] phase-2 LDS slots, out[9] heavy key limit, out[10] sub-tables of the table.
+ * n = entries to fill (at most 11). */
 int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
 
 /* ======================================================================================

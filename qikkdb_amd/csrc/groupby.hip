@@ -503,7 +503,8 @@ struct qk_groupby_s {
     uint64_t logical_cap = 0;
     int32_t reported_max = 0;
     uint64_t cap = 0;
-    int32_t parts = 1;
+    int32_t parts = 1;      // sub-tables of the hashed slots
+    int32_t rec_parts = 1;  // partitions of the two-phase path's cold rows (divides parts)
     bool real_sum = false;
     // storage (structure of arrays)
     uint64_t* keys = nullptr;
@@ -666,10 +667,10 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
         const bool defer = (h->flags & QK_GB_ASYNC) != 0 && h->tp.defer_rows != 0;
         TwoPhaseBufs& tp = h->tp;
         if (tp.pending_rows > 0 &&
-            (twophase_needs_realloc(tp, h->parts, h->value_type, values != nullptr, n, defer) ||
+            (twophase_needs_realloc(tp, h->rec_parts, h->value_type, values != nullptr, n, defer) ||
              tp.pending_rows + n > tp.rows_alloc || tp.pend_vtype != vtype))
             QK_TRY(twophase_flush(tp, g, h->fields, s));
-        QK_TRY(twophase_prepare(tp, h->parts, h->fields, h->value_type, values != nullptr, n, defer, s));
+        QK_TRY(twophase_prepare(tp, h->rec_parts, h->fields, h->value_type, values != nullptr, n, defer, s));
         if (!tp.heavy_ready)
             QK_TRY(twophase_sample(tp, g, h->ks, kc, vtype, values, value_nulls, mask, ps, vpred, n, mode, s));
         h->last_H = tp.H1;
@@ -820,20 +821,23 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     uint64_t cap = 64;
     while (cap < 2 * logical) cap <<= 1;
     h->cap = cap;
-    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only); up to
-    // 1024 of them when that lets a sub-table fit the LDS, so that the two-phase path folds
-    // straight into its sub-tables
+    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only); more
+    // of them when that lets a sub-table fit the LDS, so that the two-phase path folds
+    // straight into its sub-tables.  Cold rows go to at most kMaxRecParts partitions, each
+    // holding the rows of up to 2^kMaxSplitBits sub-tables.
     int32_t parts = 1;
     if (ks.kw == 1) {
         while (parts < 256 && cap / (uint64_t)(parts * 2) >= 1024) parts *= 2;
         const size_t budget = std::min<size_t>(device_info().lds_per_block, 160 * 1024);
         int32_t more = parts;
-        while (more < 1024 && p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) > budget &&
+        while (more < (kMaxRecParts << kMaxSplitBits) &&
+               p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) > budget &&
                cap / (uint64_t)(more * 2) >= 1024)
             more *= 2;
         if (p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) <= budget) parts = more;
     }
     h->parts = parts;
+    h->rec_parts = std::min<int32_t>(parts, kMaxRecParts);
     const size_t slots = (size_t)(cap + 2);
     int st = QK_SUCCESS;
     auto A = [&](void** p, size_t bytes) {
@@ -1081,7 +1085,7 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
     if (!h || !out || n < 0) return fail(QK_ERROR, "qk_gb_stats: bad argument");
     hipStream_t s = as_stream(stream);
     QK_HIP(hipStreamSynchronize(s));
-    int64_t v[10] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax};
+    int64_t v[11] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax, h->parts};
     if (h->tp.heavy_meta) {
         int32_t meta[2];
         QK_HIP(hipMemcpy(meta, h->tp.heavy_meta, 8, hipMemcpyDeviceToHost));
@@ -1093,7 +1097,7 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
         QK_HIP(hipMemcpy(rc.data(), h->tp.r_count, rc.size() * 4, hipMemcpyDeviceToHost));
         for (uint32_t c : rc) v[3] += c;
     }
-    for (int i = 0; i < n && i < 10; i++) out[i] = v[i];
+    for (int i = 0; i < n && i < 11; i++) out[i] = v[i];
     return QK_SUCCESS;
 }
 

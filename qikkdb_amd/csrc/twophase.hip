@@ -595,6 +595,7 @@ struct P2Args {
     const uint64_t* p_max;
     const uint32_t* r_rec;
     const uint32_t* r_count;
+    int32_t split;  // log2(sub-tables per partition), owned phase 2 only
 };
 
 __host__ __device__ inline size_t p2_lds_bytes(int32_t H2, int32_t W, uint32_t fields) {
@@ -897,7 +898,19 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
     const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
     const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
     const bool f_all = (fields & F_CNTALL) != 0;
-    const int tid = threadIdx.x, bd = blockDim.x, part = blockIdx.x;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    // this workgroup's sub-table, and the partition whose rows hold its keys.  With several
+    // sub-tables per partition (split > 0), the sub-tables of one partition go to workgroups
+    // of one die (dispatch deals workgroups to the 8 dies in turn), so the partition's rows
+    // are read from that die's L2 after the first of them.
+    int part = blockIdx.x;
+    if (a.split) {
+        const int die = blockIdx.x & 7, i = blockIdx.x >> 3;
+        part = ((((i >> a.split) << 3) + die) << a.split) | (i & ((1 << a.split) - 1));
+    }
+    const int rpart = part >> a.split;
+    // sub-table of a key hash (a two-step shift, so 64 bits of shift give 0)
+    auto sub_of = [&](uint64_t h) { return (int)((h >> 1) >> (g.part_shift - 1)); };
     const uint32_t S = (uint32_t)g.sub_mask + 1;
     const int64_t base = (int64_t)part << g.sub_bits;
     // 8-byte arrays first, then the 4-byte ones
@@ -958,7 +971,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
 
     // heavy partials of this partition's heavy keys, summed over the phase-1 workgroups (a
     // thread owns one key and a strided subset of the workgroups)
-    const int32_t hk0 = a.meta[2 + part], hk1 = a.meta[2 + part + 1];
+    const int32_t hk0 = a.meta[2 + rpart], hk1 = a.meta[2 + rpart + 1];
     const int32_t nk = hk1 - hk0;
     const int32_t stride_k = a.kmax + 2;
     if (nk > 0) {
@@ -967,6 +980,9 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
         if (tid < span) {
             const int w0 = nk <= bd ? tid / nk : 0;
             for (int ki = nk <= bd ? tid % nk : tid; ki < nk; ki += (nk <= bd ? nk : bd)) {
+                const uint64_t key = a.heavy[hk0 + ki];
+                const uint64_t kh = hash64(key);
+                if (sub_of(kh) != part) continue;
                 uint64_t rows = 0, cnt = 0, isum = 0, mn = ~0ull, mx = 0;
                 double dsum = 0.0;
 #pragma unroll 8
@@ -982,8 +998,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
                     if (omax) mx = max(mx, (uint64_t)a.p_max[at]);
                 }
                 if (!rows) continue;
-                const uint64_t key = a.heavy[hk0 + ki];
-                const int s = claim_from(key, (uint32_t)(hash64(key) & g.sub_mask));
+                const int s = claim_from(key, (uint32_t)(kh & g.sub_mask));
                 if (s >= 0) add(s, (uint32_t)rows, (uint32_t)cnt, REAL ? (uint64_t)__double_as_longlong(dsum) : isum, mn, mx);
                 else g_set_error(g, QK_HASH_TABLE_FULL);
             }
@@ -1002,8 +1017,8 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
 #pragma unroll
         for (int q = 0; q < kRegionsPerWave; q++) {
             const int w = r0 + q;
-            const int32_t c = w < a.W ? (int32_t)a.r_count[(int64_t)w * a.P + part] : 0;
-            rbase[q] = ((int64_t)w * a.P + part) * (int64_t)a.cap;
+            const int32_t c = w < a.W ? (int32_t)a.r_count[(int64_t)w * a.P + rpart] : 0;
+            rbase[q] = ((int64_t)w * a.P + rpart) * (int64_t)a.cap;
             pre[q + 1] = pre[q] + c;
         }
         const int32_t cnt = pre[kRegionsPerWave];
@@ -1021,10 +1036,15 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
             }
             // the kWindow slots from every row's home at once; a key present in the table sits
             // there unless its chain is longer (no deletions, so any match is the key's)
+            // home slot, with bit 31 set for a row of another sub-table of the partition (the
+            // window below masks the bit away)
             uint32_t home[kP2Rows];
             int hit[kP2Rows];
 #pragma unroll
-            for (int i = 0; i < kP2Rows; i++) home[i] = (uint32_t)(hash64(key[i]) & g.sub_mask);
+            for (int i = 0; i < kP2Rows; i++) {
+                const uint64_t h = hash64(key[i]);
+                home[i] = (uint32_t)(h & g.sub_mask) | ((uint32_t)(sub_of(h) != part) << 31);
+            }
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 int h = -1;
@@ -1037,7 +1057,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
             }
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
-                if (j0 + i * 64 + lane >= cnt) continue;
+                if ((j0 + i * 64 + lane >= cnt) | (home[i] >> 31)) continue;
                 Part pr;
                 if constexpr (HAS_V) pr = row_part<VT>(val[i], false); else pr = row_part<NoValue>(NoValue{}, false);
                 int s = hit[i];
@@ -1111,6 +1131,13 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
 
 // ---- launch helpers ----------------------------------------------------------------------
 
+// Shift giving a key hash's phase-1 partition among P (a power of two): h >> shift, 64 for P = 1.
+int32_t rec_shift_of(int32_t P) {
+    int32_t bits = 0;
+    while ((1 << bits) < P) bits++;
+    return 64 - bits;
+}
+
 int grid_cap(int64_t items, int per, int cap) {
     int64_t g = (items + per - 1) / per;
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
@@ -1164,7 +1191,7 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
     a.r_rec = b.r_rec;
     a.r_count = b.r_count;
     a.zero = b.zero_word;
-    a.part_shift = g.part_shift;
+    a.part_shift = rec_shift_of(b.P);
     a.accumulate = b.pending_rows > 0 ? 1 : 0;
     const size_t lds = p1_lds_bytes(b.H1, b.P, b.kmax, fields);
     k_tp_phase1<VT, MODE, KT, FP><<<b.W, b.block, lds, s>>>(b.gview, ks, kc, (const VT*)vals, vnull,
@@ -1228,10 +1255,16 @@ int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
     a.r_count = b.r_count;
     const bool sum_only = (fields & F_SUM) && !(fields & (F_MIN | F_MAX));
     const size_t olds = p2o_lds_bytes((int64_t)g.sub_mask + 1, fields);
-    if (b.P == (int32_t)(g.cap >> g.sub_bits) && olds <= std::min<size_t>(device_info().lds_per_block, 160 * 1024)) {
+    // sub-tables per partition: a power of two (both counts are), dealt to the dies by eights
+    const int64_t F = (int64_t)(g.cap >> g.sub_bits);
+    a.split = 0;
+    while (((int64_t)b.P << a.split) < F) a.split++;
+    const bool split_ok = a.split == 0 || (a.split <= kMaxSplitBits && b.P % 8 == 0);
+    if (((int64_t)b.P << a.split) == F && split_ok &&
+        olds <= std::min<size_t>(device_info().lds_per_block, 160 * 1024)) {
         // the sub-tables fit the LDS: each workgroup folds into its own sub-table directly
-        if (sum_only) k_tp_phase2_owned<VT, kFieldsSum><<<b.P, b.block, olds, s>>>(g, fields, a);
-        else k_tp_phase2_owned<VT, kFieldsAny><<<b.P, b.block, olds, s>>>(g, fields, a);
+        if (sum_only) k_tp_phase2_owned<VT, kFieldsSum><<<(int)F, b.block, olds, s>>>(g, fields, a);
+        else k_tp_phase2_owned<VT, kFieldsAny><<<(int)F, b.block, olds, s>>>(g, fields, a);
     } else if (sum_only) {
         k_tp_phase2<VT, kFieldsSum><<<b.P, b.block, p2_lds_bytes(b.H2, b.W, fields), s>>>(g, fields, a);
     } else {
@@ -1370,7 +1403,7 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
     k_tp_threshold<<<1, kHistBins, 0, s>>>(b.hist, b.kmax, b.heavy_meta);
     k_tp_collect<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots,
                                                                b.kmax, b.heavy_meta, b.cand, b.cand_cnt);
-    k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.cand_cnt, b.kmax, b.P, g.part_shift, b.heavy_meta, b.heavy_keys,
+    k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.cand_cnt, b.kmax, b.P, rec_shift_of(b.P), b.heavy_meta, b.heavy_keys,
                                  b.heavy_cnt);
     QK_HIP(hipGetLastError());
     b.heavy_ready = true;
@@ -1386,7 +1419,8 @@ int twophase_phase1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
         return fail(QK_ERROR, "group by: value column type changed between blocks");
     if (b.pending_rows + n > b.rows_alloc || n > b.block_alloc)
         return fail(QK_ERROR, "group by: two-phase buffers too small for the block");
-    if (g.part_shift < 33 || g.part_shift > 64)
+    if (g.part_shift < 33 || g.part_shift > 64 || b.P > kMaxRecParts || (b.P & (b.P - 1)) ||
+        (int64_t)(g.cap >> g.sub_bits) < b.P)
         return fail(QK_ERROR, "group by: unsupported partition count for the two-phase path");
     if (!b.gview_valid || memcmp(&b.gview_host, &g, sizeof(GView)) != 0) {
         b.gview_host = g;

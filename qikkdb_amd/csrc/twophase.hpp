@@ -71,6 +71,12 @@ __host__ __device__ inline size_t p2o_lds_bytes(int64_t S, uint32_t fields) {
     return (size_t)S * (size_t)owned_slot_bytes(fields);
 }
 
+// Phase 1 writes cold rows to at most kMaxRecParts partitions (the top bits of the key hash).
+// A table may have up to 2^kMaxSplitBits sub-tables per partition: phase 2 then runs one
+// workgroup per sub-table, each reading its partition's rows and keeping its own.
+constexpr int32_t kMaxRecParts = 1024;
+constexpr int32_t kMaxSplitBits = 4;
+
 // Whether the two-phase path applies: single-word keys and a value type it handles.
 bool twophase_supported(int32_t kw, int32_t value_type);
 

@@ -36,9 +36,10 @@ class Where:
 
 def gpu_group_by(aggs, keys, value=None, key_nulls=None, value_nulls=None, where=(), mask=None,
                  block=None, capacity=1 << 16, flags=0, tuning=None, out_types=None,
-                 extra_cols=None, value_type=None, path=None, defer_rows=None):
+                 extra_cols=None, value_type=None, path=None, defer_rows=None, stats=None):
     """keys: list of numpy arrays; key_nulls / value_nulls: bool arrays or None.
-    Rows are processed in blocks of `block` rows (each block uploaded with its own masks)."""
+    Rows are processed in blocks of `block` rows (each block uploaded with its own masks).
+    A dict passed as `stats` receives the table's tuning state after the results."""
     keys = [np.ascontiguousarray(k) for k in keys]
     n = keys[0].size
     ktypes = [qk._abi.TYPE_OF[k.dtype] for k in keys]
@@ -75,6 +76,8 @@ def gpu_group_by(aggs, keys, value=None, key_nulls=None, value_nulls=None, where
         else:
             gb.ProcessBlock([kc.data for kc in kcols], vptr, e - s, knm, vnm)
     res = gb.results_host()
+    if stats is not None:
+        stats.update(gb.stats())
     gb.close()
     return res
 

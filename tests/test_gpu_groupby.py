@@ -311,12 +311,30 @@ def test_two_phase_uniform_high_cardinality_and_sentinel():
     compare(gpu, cpu, 4)
 
 
+def expected_layout(capacity, sum_only):
+    """(sub-tables, cold-row partitions) of a two-phase table: sub-tables of at most 160 KB of
+    owned LDS state (20 B per slot for SUM/AVG/COUNT, 36 B with MIN/MAX), at most 1024
+    partitions of up to 16 sub-tables each; otherwise 256 sub-tables, one per partition."""
+    cap = 64
+    while cap < 2 * capacity:
+        cap *= 2
+    per_slot = 20 if sum_only else 36
+    s = 1
+    while 2 * s * per_slot <= 160 * 1024:
+        s *= 2
+    f = max(256, cap // s)
+    return (f, min(f, 1024)) if f <= 16384 else (256, 256)
+
+
 @pytest.mark.parametrize("flags", [0, qk.GB_ASYNC])
-@pytest.mark.parametrize("sum_only", [False, True])
-def test_two_phase_merge_path_for_large_tables(flags, sum_only):
-    """A table of 2^24 slots has sub-tables too large for the LDS even at 1024 of them, so
-    phase 2 aggregates in an LDS hash table and merges into its own sub-table (the path of
-    the 16.7M-group composite workload); heavy hitters, a long tail and NULL keys."""
+@pytest.mark.parametrize("sum_only,capacity", [(False, 5_000_000), (True, 5_000_000), (True, 40_000_000),
+                                               (False, 40_000_000), (True, 100_000_000)])
+def test_two_phase_large_tables(flags, sum_only, capacity):
+    """Tables whose 256 sub-tables are too large for the LDS.  Up to 16 sub-tables per
+    cold-row partition, phase 2 runs one workgroup per sub-table that keeps its own rows of the
+    partition (2^24 slots: 2 or 4 per partition; 2^27 slots of SUM/AVG/COUNT: 16); beyond that
+    it aggregates in an LDS hash table and merges into the partition's sub-table (2^27 slots
+    with MIN/MAX, 2^28 slots).  Heavy hitters, a long tail and NULL keys."""
     rng = np.random.default_rng(91)
     n = 2_000_000
     keys = np.where(rng.random(n) < 0.5, datagen.zipf_keys(13, 1.1, 5000, n),
@@ -328,7 +346,10 @@ def test_two_phase_merge_path_for_large_tables(flags, sum_only):
     else:
         vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
         aggs, real = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT], ()
-    gpu = gpu_group_by(aggs, [keys], vals, kn, None, block=1 << 19, capacity=5_000_000, path=1, flags=flags)
+    st = {}
+    gpu = gpu_group_by(aggs, [keys], vals, kn, None, block=1 << 19, capacity=capacity, path=1, flags=flags,
+                       stats=st)
+    assert (st["sub_tables"], st["partitions"]) == expected_layout(capacity, sum_only)
     cpu = cpu_group_by(aggs, [keys], vals, kn, None)
     compare(gpu, cpu, len(aggs), real_aggs=real)
 
