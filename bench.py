@@ -118,8 +118,10 @@ class Composite:
 
     def table(self):
         qk = self.qk
-        return qk.GPUGroupBy(self.aggs, [qk.INT, qk.INT], qk.LONG, 1 << 24, 1,
-                             flags=qk.GB_KEYS_NOT_NULL | qk.GB_ASYNC)
+        t = qk.GPUGroupBy(self.aggs, [qk.INT, qk.INT], qk.LONG, 1 << 24, 1,
+                          flags=qk.GB_KEYS_NOT_NULL | qk.GB_ASYNC)
+        t.set_defer_rows(self.n)  # the query's 8 blocks share one phase 2 (about 13 GB of regions)
+        return t
 
     def block(self, off, blk):
         qk = self.qk

@@ -925,13 +925,26 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
     uint32_t* ocnt = (uint32_t*)p;
     uint32_t* oall = f_all ? ocnt + S : nullptr;
 
-    for (uint32_t s = tid; s < S; s += bd) {
-        okey[s] = g.keys[base + s];
-        ocnt[s] = 0;
-        if (oall) oall[s] = 0;
-        if (osum) osum[s] = 0;
-        if (omin) omin[s] = ~0ull;
-        if (omax) omax[s] = 0;
+    // kFill slots per thread at a time, their loads issued together
+    constexpr int kFill = 4;
+    for (uint32_t s0 = tid; s0 < S; s0 += kFill * bd) {
+        uint64_t k[kFill];
+#pragma unroll
+        for (int u = 0; u < kFill; u++) {
+            const uint32_t s = s0 + u * bd;
+            k[u] = s < S ? g.keys[base + s] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kFill; u++) {
+            const uint32_t s = s0 + u * bd;
+            if (s >= S) continue;
+            okey[s] = k[u];
+            ocnt[s] = 0;
+            if (oall) oall[s] = 0;
+            if (osum) osum[s] = 0;
+            if (omin) omin[s] = ~0ull;
+            if (omax) omax[s] = 0;
+        }
     }
     __syncthreads();
 
@@ -1070,21 +1083,43 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
     __syncthreads();
     // every key goes back (a claimed key may carry only NULL-value rows); states where a slot
     // took rows
-    for (uint32_t s = tid; s < S; s += bd) {
-        g.keys[base + s] = okey[s];
-        const uint32_t c = ocnt[s], r = oall ? oall[s] : 0u;
-        if (r) g.cntall[base + s] += r;
-        if (!c) continue;
-        g.cnt[base + s] += c;
-        if (osum) {
-            if (REAL)
-                g.sum[base + s] = (unsigned long long)__double_as_longlong(
-                    __longlong_as_double((long long)g.sum[base + s]) + __longlong_as_double((long long)osum[s]));
-            else
-                g.sum[base + s] += osum[s];
+    // (kFill slots per thread at a time: every global state they need is loaded before any
+    // is stored)
+    for (uint32_t s0 = tid; s0 < S; s0 += kFill * bd) {
+        uint32_t c[kFill], r[kFill];
+        unsigned long long gall[kFill], gcnt[kFill], gsum[kFill], gmn[kFill], gmx[kFill];
+#pragma unroll
+        for (int u = 0; u < kFill; u++) {
+            const uint32_t s = s0 + u * bd;
+            const bool in = s < S;
+            c[u] = in ? ocnt[s] : 0u;
+            r[u] = in && oall ? oall[s] : 0u;
+            const int64_t gs = base + (in ? s : 0u);
+            gall[u] = r[u] ? g.cntall[gs] : 0ull;
+            gcnt[u] = c[u] ? g.cnt[gs] : 0ull;
+            gsum[u] = c[u] && osum ? g.sum[gs] : 0ull;
+            gmn[u] = c[u] && omin ? g.mn[gs] : 0ull;
+            gmx[u] = c[u] && omax ? g.mx[gs] : 0ull;
         }
-        if (omin && omin[s] < g.mn[base + s]) g.mn[base + s] = omin[s];
-        if (omax && omax[s] > g.mx[base + s]) g.mx[base + s] = omax[s];
+#pragma unroll
+        for (int u = 0; u < kFill; u++) {
+            const uint32_t s = s0 + u * bd;
+            if (s >= S) continue;
+            const int64_t gs = base + s;
+            g.keys[gs] = okey[s];
+            if (r[u]) g.cntall[gs] = gall[u] + r[u];
+            if (!c[u]) continue;
+            g.cnt[gs] = gcnt[u] + c[u];
+            if (osum) {
+                if (REAL)
+                    g.sum[gs] = (unsigned long long)__double_as_longlong(
+                        __longlong_as_double((long long)gsum[u]) + __longlong_as_double((long long)osum[s]));
+                else
+                    g.sum[gs] = gsum[u] + osum[s];
+            }
+            if (omin && omin[s] < gmn[u]) g.mn[gs] = omin[s];
+            if (omax && omax[s] > gmx[u]) g.mx[gs] = omax[s];
+        }
     }
     // NULL key and key == kEmpty: side slots, summed over the phase-1 workgroups (a strided
     // share per thread, then a sum per wavefront added with device atomics)
