@@ -1019,32 +1019,46 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
     }
 
     // cold rows: every wave walks the partition's regions of kRegionsPerWave phase-1
-    // workgroups at a time as one run of rows
+    // workgroups at a time as one run of rows.  The wave index is read into a scalar register,
+    // so the run's counts and offsets are scalar; the next run's counts are loaded while this
+    // run is walked.
     constexpr int kRegionsPerWave = 4;
     constexpr int kWindow = 4;
-    const int lane = tid & 63, wave = tid >> 6, nwaves = bd >> 6;
-    for (int r0 = wave * kRegionsPerWave; r0 < a.W; r0 += nwaves * kRegionsPerWave) {
-        int64_t rbase[kRegionsPerWave];
-        int32_t pre[kRegionsPerWave + 1];
-        pre[0] = 0;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = bd >> 6;
+    const int32_t stride = a.P * a.cap;  // between the regions of one partition (host-checked)
+    int32_t cn[kRegionsPerWave];
+    auto counts = [&](int r0) {
 #pragma unroll
         for (int q = 0; q < kRegionsPerWave; q++) {
             const int w = r0 + q;
-            const int32_t c = w < a.W ? (int32_t)a.r_count[(int64_t)w * a.P + rpart] : 0;
-            rbase[q] = ((int64_t)w * a.P + rpart) * (int64_t)a.cap;
-            pre[q + 1] = pre[q] + c;
+            cn[q] = w < a.W ? (int32_t)a.r_count[(int64_t)w * a.P + rpart] : 0;
         }
-        const int32_t cnt = pre[kRegionsPerWave];
+    };
+    counts(wave * kRegionsPerWave);
+    for (int r0 = wave * kRegionsPerWave; r0 < a.W; r0 += nwaves * kRegionsPerWave) {
+        // row j of the run lies in region q = #{q' >= 1 : j >= pre[q']}, at record
+        // rbase + j + the jumps of those q' (a jump skips the unused rest of a region)
+        const int64_t rbase = ((int64_t)r0 * a.P + rpart) * (int64_t)a.cap;
+        int32_t pre[kRegionsPerWave], jump[kRegionsPerWave];
+        pre[0] = 0;
+        jump[0] = 0;
+#pragma unroll
+        for (int q = 1; q < kRegionsPerWave; q++) {
+            pre[q] = pre[q - 1] + cn[q - 1];
+            jump[q] = stride - cn[q - 1];
+        }
+        const int32_t cnt = pre[kRegionsPerWave - 1] + cn[kRegionsPerWave - 1];
+        counts(r0 + nwaves * kRegionsPerWave);
         for (int32_t j0 = 0; j0 < cnt; j0 += 64 * kP2Rows) {
             uint64_t key[kP2Rows];
             typename std::conditional<HAS_V, VT, int>::type val[kP2Rows];
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 const int32_t j = min(j0 + i * 64 + lane, cnt - 1);
-                int64_t at = rbase[0] + j;
+                int32_t off = j;
 #pragma unroll
-                for (int q = 1; q < kRegionsPerWave; q++)
-                    if (j >= pre[q]) at = rbase[q] + (j - pre[q]);
+                for (int q = 1; q < kRegionsPerWave; q++) off += j >= pre[q] ? jump[q] : 0;
+                const int64_t at = rbase + off;
                 get_rec<VT>(a.r_rec, at, key[i], val[i]);
             }
             // the kWindow slots from every row's home at once; a key present in the table sits
@@ -1294,7 +1308,9 @@ int launch_p2(TwoPhaseBufs& b, const GView& g, uint32_t fields, hipStream_t s) {
     const int64_t F = (int64_t)(g.cap >> g.sub_bits);
     a.split = 0;
     while (((int64_t)b.P << a.split) < F) a.split++;
-    const bool split_ok = a.split == 0 || (a.split <= kMaxSplitBits && b.P % 8 == 0);
+    // (the owned kernel addresses a run of regions of one partition with 32-bit offsets)
+    const bool split_ok = (a.split == 0 || (a.split <= kMaxSplitBits && b.P % 8 == 0)) &&
+                          (int64_t)b.P * b.cap * 8 < ((int64_t)1 << 31);
     if (((int64_t)b.P << a.split) == F && split_ok &&
         olds <= std::min<size_t>(device_info().lds_per_block, 160 * 1024)) {
         // the sub-tables fit the LDS: each workgroup folds into its own sub-table directly
