@@ -320,11 +320,15 @@ __device__ __forceinline__ int heavy_find(const uint64_t* lkey, uint32_t nb, uin
     return s;
 }
 
-// KT is the element type of a single integer key column, or void for any key layout (read
-// through the key spec's runtime switch).  With a typed column, the loads of a whole tile are
-// issued before any of them is used.
+// KT is the element type of a single integer key column, KeyPairI32 for two non-NULL int32
+// key columns packed into one word (the first in the low half), or void for any key layout
+// (read through the key spec's runtime switch).  With a typed key, the loads of a whole tile
+// are issued before any of them is used.
+struct KeyPairI32 {};
 template <typename KT>
-constexpr bool kTypedKey = !std::is_same<KT, void>::value;
+constexpr bool kPairKey = std::is_same<KT, KeyPairI32>::value;
+template <typename KT>
+constexpr bool kTypedKey = !std::is_same<KT, void>::value && !kPairKey<KT>;
 
 // The global table is read through a pointer: only the rare atomic path needs it, and kernel
 // arguments kept in scalar registers for the whole loop would spill.
@@ -451,6 +455,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                 // streamed once: non-temporal, so that the lines of the cold regions being
                 // filled stay in the L2 instead of being pushed out by the column reads
                 t.key[k] = (uint64_t)(int64_t)__builtin_nontemporal_load((const KT*)kc.col[0] + r);
+            } else if constexpr (kPairKey<KT>) {
+                // the key spec's packing of two int32 fields at bits 0 and 32 (no null bits)
+                const uint32_t lo = (uint32_t)__builtin_nontemporal_load((const int32_t*)kc.col[0] + r);
+                const uint32_t hi = (uint32_t)__builtin_nontemporal_load((const int32_t*)kc.col[1] + r);
+                t.key[k] = (uint64_t)lo | ((uint64_t)hi << 32);
             } else {
                 uint64_t kk[1];
                 load_key<1>(ks, kc, r, kk, t.kn[k]);
@@ -1282,6 +1291,9 @@ int launch_p1_k(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, co
             return launch_p1_m<VT, int32_t, kFieldsSum>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
         return launch_p1_m<VT, int32_t, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     }
+    if (!ks.single && ks.nkeys == 2 && ks.kw == 1 && ks.type[0] == QK_INT && ks.type[1] == QK_INT &&
+        ks.word[0] == 0 && ks.shift[0] == 0 && ks.word[1] == 0 && ks.shift[1] == 32 && !kc.nm[0] && !kc.nm[1])
+        return launch_p1_m<VT, KeyPairI32, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     return launch_p1_m<VT, void, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
 }
 

@@ -184,8 +184,11 @@ def test_type_rules_rejected_like_reference():
 
 # ---- composite keys ----------------------------------------------------------------------------
 
+@pytest.mark.parametrize("path", [None, 1])
 @pytest.mark.parametrize("flags", [0, qk.GB_KEYS_NOT_NULL])
-def test_composite_int_int(flags):
+def test_composite_int_int(flags, path):
+    # non-NULL (int32, int32) keys pack into one word and, forced to the two-phase path, are
+    # read by its two-column key loads; negative second fields fill the word's high half
     rng = np.random.default_rng(flags + 3)
     n = 600_000
     a = rng.integers(0, 300, n).astype(np.int32)
@@ -193,8 +196,31 @@ def test_composite_int_int(flags):
     vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
     kn = None if flags else [rng.random(n) < 0.05, rng.random(n) < 0.05]
     aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT, qk.AGG_AVG]
-    gpu = gpu_group_by(aggs, [a, b], vals, kn, capacity=200_000, flags=flags)
+    gpu = gpu_group_by(aggs, [a, b], vals, kn, capacity=200_000, flags=flags, path=path)
     cpu = cpu_group_by(aggs, [a, b], vals, kn)
+    compare(gpu, cpu, len(aggs))
+
+
+@pytest.mark.parametrize("mode", ["mask", "value_range", "mixed_where"])
+def test_composite_pair_key_two_phase_filters(mode):
+    rng = np.random.default_rng(17)
+    n = 900_000
+    a = rng.integers(-2**31, 2**31, n).astype(np.int32) % 5000
+    b = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    b[::7] = rng.integers(-50, 50, b[::7].size)  # some keys repeat
+    vals = rng.integers(-10**6, 10**6, n).astype(np.int32)
+    aggs = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT]
+    kw = dict(capacity=1_000_000, flags=qk.GB_KEYS_NOT_NULL, block=1 << 18)
+    if mode == "mask":
+        m = (rng.random(n) < 0.4).astype(np.int8)
+        gpu = gpu_group_by(aggs, [a, b], vals, mask=m, path=1, **kw)
+        cpu = cpu_group_by(aggs, [a, b], vals, mask=m)
+    else:
+        w = [Where(qk.GREATER, "value", -200_000, qk.INT), Where(qk.LESS_EQUAL, "value", 500_000, qk.INT)]
+        if mode == "mixed_where":
+            w.append(Where(qk.NOT_EQUAL, "key0", 3, qk.INT))
+        gpu = gpu_group_by(aggs, [a, b], vals, where=w, path=1, **kw)
+        cpu = cpu_group_by(aggs, [a, b], vals, where=w)
     compare(gpu, cpu, len(aggs))
 
 
