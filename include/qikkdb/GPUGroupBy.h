@@ -3,12 +3,19 @@
 // composite-key form GPUGroupBy<AGG,O,std::vector<void*>,V> (GPUGroupByMultiKey.cuh:282-923).
 // Each table forwards to one qk_groupby handle of the C ABI.
 //
-// Not provided: the pre-seeded constructor (GPUGroupBy.cuh:367-372) and the
-// GetResults(..., tables, ...) / ReconstructRawNumbers overloads (:433-790), which exist for
-// the reference's host-staged merge of one table per GPU; this build runs on one GPU.
+// GetResults(..., tables, ...) (GPUGroupBy.cuh:593-790, GPUGroupByMultiKey.cuh:924-1324) is the
+// only GetResults the reference's dispatcher calls (GpuSqlDispatcherAggregationFunctions.h:141,
+// :203, :303).  With one table it is the single-table GetResults, as in the reference; with
+// several tables (one per dispatcher thread) the others are folded into this one on the device
+// (qk_gb_merge) instead of being staged through the host.  Tables on different GPUs would need
+// peer access, which this one-GPU build does not set up: that case throws.
+// The composite form has no ReconstructRawNumbers (its reference version fills host key
+// columns of every type, strings included, for the host-staged merge only).
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <stdexcept>
 #include <type_traits>
 #include <vector>
 
@@ -26,6 +33,20 @@ public:
 };
 
 namespace qikkdb {
+
+// Fold the dispatcher's other tables into `self` (the multi-table GetResults).  As in the
+// reference, an empty vector is an error and a vector whose second entry is null holds one table.
+template <typename T>
+void merge_tables(T* self, qk_groupby self_h, std::vector<std::unique_ptr<IGroupBy>>& tables) {
+    if (tables.size() == 1 || (tables.size() > 1 && tables[1].get() == nullptr)) return;
+    for (auto& t : tables) {
+        if (!t || t.get() == self) continue;
+        T* other = dynamic_cast<T*>(t.get());
+        if (!other)
+            throw query_engine_error(GPU_EXTENSION_ERROR, "GetResults: the tables are not of the same GPUGroupBy type");
+        check(qk_gb_merge(self_h, other->handle(), nullptr));
+    }
+}
 
 // The typing rules of GPUGroupBy.cuh:522-529 and :563-567: COUNT gives int64, every other
 // aggregate gives the value type (this build also accepts a widened SUM: int32 -> int64,
@@ -76,6 +97,18 @@ public:
         qikkdb::check_output_type<AGG, O, V>();
     }
 
+    // A table holding the maxHashCount keys of the device buffer `keys` (keys[0] stands for
+    // the NULL key when firstKeyIsNull) as groups from the start; GetResults returns them first,
+    // in the order of `keys` (GPUGroupBy.cuh:367-372, where they fill consecutive slots).
+    GPUGroupBy(const int32_t maxHashCount, K* keys, bool firstKeyIsNull)
+    : table_(1, key_type(), qikkdb::TypeCode<V>::value, AGG::code, qikkdb::TypeCode<O>::value,
+             maxHashCount - (firstKeyIsNull ? 1 : 0) > 0 ? maxHashCount - (firstKeyIsNull ? 1 : 0) : 1, 1) {
+        qikkdb::check_output_type<AGG, O, V>();
+        qikkdb::check(qk_gb_seed_keys(table_.get(), keys, maxHashCount, firstKeyIsNull ? 1 : 0, nullptr));
+    }
+
+    qk_groupby handle() const { return table_.get(); }
+
     // Aggregate one block; NULL masks may be nullptr.  Waits for the kernel and throws
     // query_engine_error(GPU_HASH_TABLE_FULL) when the table filled up.
     void ProcessBlock(K* inKeys, V* inValues, int32_t dataElementCount, nullmask_t* inKeysNullMask = nullptr,
@@ -108,6 +141,59 @@ public:
         if (outKeysNullMask) *outKeysNullMask = knulls[0];
         if (outValuesNullMask) *outValuesNullMask = vnulls[0];
     }
+
+    // The dispatcher's GetResults over the tables of all its threads (GPUGroupBy.cuh:593-790).
+    void GetResults(K** outKeys, O** outValues, int32_t* outDataElementCount,
+                    std::vector<std::unique_ptr<IGroupBy>>& tables, nullmask_t** outKeysNullMask = nullptr,
+                    nullmask_t** outValuesNullMask = nullptr) {
+        if (tables.size() <= 0) throw std::invalid_argument("Number of tables have to be at least 1.");
+        qikkdb::merge_tables(this, table_.get(), tables);
+        GetResults(outKeys, outValues, outDataElementCount, outKeysNullMask, outValuesNullMask);
+    }
+
+    // Raw fields before finalisation, copied to HOST buffers of at least GetMaxHashCount()
+    // entries (GPUGroupBy.cuh:433-471): keys and one NULL byte per key; for MIN/MAX/SUM/AVG the
+    // merged values (the sum for AVG) and one NULL byte per value; for AVG/COUNT the row counts.
+    void ReconstructRawNumbers(K* keys, nullarray_t* keysNullArray, V* values, nullarray_t* valuesNullArray,
+                               int64_t* occurrences, int32_t* elementCount) {
+        constexpr bool OCC = std::is_same<AGG, AggregationFunctions::avg>::value ||
+                             std::is_same<AGG, AggregationFunctions::count>::value;
+        const int32_t vagg = std::is_same<AGG, AggregationFunctions::avg>::value ? AggregationFunctions::sum::code
+                                                                                  : AGG::code;
+        int32_t aggs[2] = {USE_VALUES ? vagg : QK_AGG_NONE, QK_AGG_COUNT};
+        int32_t otypes[2] = {qikkdb::TypeCode<V>::value, QK_LONG};
+        void* k[1] = {nullptr};
+        uint64_t* kn[1] = {nullptr};
+        void* v[2] = {nullptr, nullptr};
+        uint64_t* vn[2] = {nullptr, nullptr};
+        int32_t g = 0;
+        qikkdb::check(qk_gb_get_results_aggs(table_.get(), 2, aggs, otypes, k, kn, v, vn, &g, nullptr));
+        cuda_ptr_set owned{{k[0], kn[0], v[0], v[1], vn[0], vn[1]}};
+        *elementCount = g;
+        if (g == 0) return;
+        std::vector<uint64_t> bits(NullValues::GetNullBitMaskSize(g));
+        auto unpack = [&](const uint64_t* dbits, nullarray_t* out) {
+            GPUMemory::copyDeviceToHost(bits.data(), dbits, bits.size());
+            for (int32_t i = 0; i < g; i++) out[i] = (nullarray_t)((bits[i >> 6] >> (i & 63)) & 1ull);
+        };
+        GPUMemory::copyDeviceToHost(keys, static_cast<K*>(k[0]), g);
+        if (keysNullArray) unpack(kn[0], keysNullArray);
+        if (USE_VALUES) {
+            if (values) GPUMemory::copyDeviceToHost(values, static_cast<V*>(v[0]), g);
+            if (valuesNullArray) unpack(vn[0], valuesNullArray);
+        }
+        if (OCC && occurrences) GPUMemory::copyDeviceToHost(occurrences, static_cast<int64_t*>(v[1]), g);
+    }
+
+private:
+    // frees the device buffers of one results read
+    struct cuda_ptr_set {
+        std::vector<void*> p;
+        ~cuda_ptr_set() {
+            for (void* x : p)
+                if (x) qk_free(x);
+        }
+    };
 };
 
 template <typename AGG, typename O, typename V>
@@ -131,6 +217,8 @@ public:
              qikkdb::TypeCode<O>::value, maxHashCount, keySizeMultiplier) {
         qikkdb::check_output_type<AGG, O, V>();
     }
+
+    qk_groupby handle() const { return table_.get(); }
 
     void ProcessBlock(std::vector<void*> inKeysVector, std::vector<nullmask_t*> inKeysNullMaskVector, V* inValues,
                       int32_t dataElementCount, nullmask_t* inValuesNullMask = nullptr) {
@@ -163,5 +251,14 @@ public:
         if (outKeysNullMasksVector) outKeysNullMasksVector->assign(knulls.begin(), knulls.end());
         *outValues = static_cast<O*>(vals[0]);
         if (outValuesNullMask) *outValuesNullMask = vnulls[0];
+    }
+
+    // The dispatcher's GetResults over the tables of all its threads (GPUGroupByMultiKey.cuh:924-1324).
+    void GetResults(std::vector<void*>* outKeysVector, O** outValues, int32_t* outDataElementCount,
+                    std::vector<std::unique_ptr<IGroupBy>>& tables,
+                    std::vector<nullmask_t*>* outKeysNullMasksVector = nullptr, nullmask_t** outValuesNullMask = nullptr) {
+        if (tables.size() <= 0) CheckQueryEngineError(GPU_EXTENSION_ERROR, "Number of tables have to be at least 1.");
+        qikkdb::merge_tables(this, table_.get(), tables);
+        GetResults(outKeysVector, outValues, outDataElementCount, outKeysNullMasksVector, outValuesNullMask);
     }
 };

@@ -82,6 +82,9 @@ const char* qk_last_error_message(void);
 /* Pooled device allocation.  Replaces GPUMemory::alloc / GPUMemory::free (GPUMemory.cuh:89-110,
  * GPUMemory.cpp:15-20) over the reference's CudaMemAllocator (CudaMemAllocator.cu:109-322). */
 int qk_malloc(void** ptr, size_t bytes);
+/* The block goes back to the pool at once, and the next qk_malloc of its size may return it,
+ * to any stream: free a buffer only after the work that reads it has completed (the
+ * reference's GPUMemory::free is cudaFree, which waits for the device itself). */
 int qk_free(void* ptr);
 /* Bytes handed out and bytes held in the pool's free lists. */
 int qk_mem_stats(size_t* in_use, size_t* cached);
@@ -258,6 +261,33 @@ int qk_gb_check_error(qk_groupby h, void* stream);
 int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls,
                       void** out_values, uint64_t** out_value_nulls, int32_t* out_count,
                       void* stream);
+
+/* As qk_gb_get_results, with other outputs computed from the aggregate states the table keeps:
+ * aggs[naggs] may name any aggregate whose state exists (COUNT always; SUM and AVG when the table
+ * keeps a sum, i.e. it was created with SUM or AVG; MIN; MAX; COUNT_ALL), out_types as in
+ * qk_gb_create (NULL or 0 = reference typing).  This is how the raw numbers of
+ * GPUGroupBy::ReconstructRawNumbers (GPUGroupBy.cuh:433-471: merged sums and occurrence counts
+ * of an AVG table, etc.) are read. */
+int qk_gb_get_results_aggs(qk_groupby h, int32_t naggs, const int32_t* aggs, const int32_t* out_types,
+                           void** out_keys, uint64_t** out_key_nulls, void** out_values,
+                           uint64_t** out_value_nulls, int32_t* out_count, void* stream);
+
+/* Pre-seed a fresh single-key table with n distinct keys (a device column of the table's key
+ * type; key 0 stands for the NULL group when first_key_is_null): each becomes a group even if
+ * no row reaches it (COUNT 0, other aggregates NULL), and qk_gb_get_results returns the
+ * seeded groups first, in the order of `keys`, then any other group.  Replaces the
+ * constructor GPUGroupBy(maxHashCount, K* keys, bool firstKeyIsNull) (GPUGroupBy.cuh:367-372),
+ * whose table keeps its keys at consecutive slots so that GetResults follows their order.
+ * QK_ERROR when the table already holds groups or the keys are not distinct. */
+int qk_gb_seed_keys(qk_groupby h, const void* keys, int32_t n, int32_t first_key_is_null, void* stream);
+
+/* Fold every group of table src into table dst (tables of the same key and aggregate layout on
+ * the calling thread's device): afterwards dst holds what one table that saw both tables' rows
+ * would hold; src is unchanged.  Replaces the multi-table GetResults(..., tables, ...) merge
+ * (GPUGroupBy.cuh:594-790, GPUGroupByMultiKey.cuh:924-1324), which copies every table's
+ * results to the host and re-aggregates them there; here the states (count, sum, min, max) are
+ * merged on the device, so AVG stays exact.  Waits; QK_HASH_TABLE_FULL when dst fills up. */
+int qk_gb_merge(qk_groupby dst, qk_groupby src, void* stream);
 
 /* Number of groups currently in the table (waits for the table's work). */
 int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream);

@@ -233,3 +233,72 @@ def query_mt(nthreads, aggs, key_types, value_type, keys, values=None, value_nul
     if not h:
         raise RuntimeError("oracle threaded run failed")
     return GroupBy(aggs, key_types, value_type, out_types, handle=h)
+
+
+def _key_bits(k: np.ndarray) -> np.ndarray:
+    """Integer view of a key column for sorting; float keys with -0.0 folded into +0.0."""
+    k = np.asarray(k)
+    if np.issubdtype(k.dtype, np.floating):
+        k = k + k.dtype.type(0)  # -0.0 + 0.0 == +0.0; NaN payloads are kept
+        return k.view(np.int32 if k.dtype.itemsize == 4 else np.int64)
+    return k
+
+
+def sort_groups(res: dict) -> dict:
+    """The groups of a results dict (oracle or GPU) ordered by key, NULL keys first within each
+    column, so that two results compare row by row whatever their slot order (S10)."""
+    g = res["count"]
+    nk = len(res["keys"])
+    cols = []
+    for c in range(nk):
+        kn = np.asarray(res["key_nulls"][c], bool)[:g]
+        kb = np.where(kn, 0, _key_bits(res["keys"][c][:g]))
+        cols += [kb, ~kn]
+    order = np.lexsort(cols[::-1]) if g else np.zeros(0, np.int64)
+    out = {"count": g,
+           "keys": [np.where(np.asarray(res["key_nulls"][c], bool)[:g], 0, _key_bits(res["keys"][c][:g]))[order]
+                    for c in range(nk)],
+           "key_nulls": [np.asarray(res["key_nulls"][c], bool)[:g][order] for c in range(nk)],
+           "values": [], "value_nulls": []}
+    for a in range(len(res["values"])):
+        v = res["values"][a]
+        vn = res["value_nulls"][a]
+        out["values"].append(None if v is None else np.asarray(v)[:g][order])
+        out["value_nulls"].append(np.zeros(g, bool) if vn is None else np.asarray(vn, bool)[:g][order])
+    return out
+
+
+def compare_groups(got: dict, want: dict, aggs=None, rel_aggs=(), rel: float = 0.0) -> None:
+    """Every group of `got` equals the group of `want` with the same key: the same key set,
+    the same NULL flags, values bit-exact (integers) or within `rel` relative (aggregates in
+    rel_aggs).  Vectorised, for results of millions of groups."""
+    assert got["count"] == want["count"], f"group counts differ: {got['count']} vs {want['count']}"
+    a, b = sort_groups(got), sort_groups(want)
+    for c in range(len(a["keys"])):
+        bad = np.flatnonzero((a["keys"][c] != b["keys"][c]) | (a["key_nulls"][c] != b["key_nulls"][c]))
+        assert bad.size == 0, f"key column {c}: {bad.size} groups differ, first at sorted row {bad[0]}"
+    naggs = len(b["values"]) if aggs is None else len(aggs)
+    for i in range(naggs):
+        if b["values"][i] is None:
+            continue
+        vn_a, vn_b = a["value_nulls"][i], b["value_nulls"][i]
+        bad = np.flatnonzero(vn_a != vn_b)
+        assert bad.size == 0, f"aggregate {i}: NULL flags differ in {bad.size} groups"
+        live = ~vn_b
+        x = np.asarray(a["values"][i])[live]
+        y = np.asarray(b["values"][i])[live]
+        if i in rel_aggs and rel > 0:
+            x = x.astype(np.float64)
+            y = y.astype(np.float64)
+            err = np.abs(x - y) > rel * np.maximum(np.abs(y), 1e-30)
+            bad = np.flatnonzero(err)
+            assert bad.size == 0, (f"aggregate {i}: {bad.size} groups outside rel {rel}, "
+                                   f"e.g. {x[bad[0]]} vs {y[bad[0]]}")
+        elif np.issubdtype(x.dtype, np.floating) or np.issubdtype(y.dtype, np.floating):
+            x = x.astype(np.float64)
+            y = y.astype(np.float64)
+            bad = np.flatnonzero((x != y) & ~(np.isnan(x) & np.isnan(y)))
+            assert bad.size == 0, f"aggregate {i}: {bad.size} groups differ, e.g. {x[bad[0]]} vs {y[bad[0]]}"
+        else:
+            bad = np.flatnonzero(x.astype(np.int64) != y.astype(np.int64))
+            assert bad.size == 0, f"aggregate {i}: {bad.size} groups differ, e.g. {x[bad[0]]} vs {y[bad[0]]}"

@@ -179,6 +179,9 @@ PROTOTYPES = {
     "qk_gb_set_path": [_vp, _i32, _i64],
     "qk_gb_set_defer_rows": [_vp, _i64],
     "qk_gb_stats": [_vp, _P(_i64), _i32, _vp],
+    "qk_gb_seed_keys": [_vp, _vp, _i32, _i32, _vp],
+    "qk_gb_merge": [_vp, _vp, _vp],
+    "qk_gb_get_results_aggs": [_vp, _i32, _P(_i32), _P(_i32), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_i32), _vp],
     "qk_batch_create": [_P(_vp), _P(BatchColumns)],
     "qk_batch_destroy": [_vp],
     "qk_batch_set_tuning": [_vp, _P(BatchTuning)],

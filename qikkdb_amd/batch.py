@@ -111,8 +111,9 @@ class BatchEngine:
         offs = np.ctypeslib.as_array(C.cast(p[4], C.POINTER(C.c_int64)), shape=(self.last + 1,)).copy()
         return p[0].value, p[1].value, p[2].value, p[3].value, offs
 
-    def result_host(self, i: int) -> dict:
-        """Groups of query i of the last run: key -> (sum, min, max, count)."""
+    def result_arrays(self, i: int) -> list:
+        """Dense result arrays of query i of the last run, [count, sum, min, max], each of
+        key_mod int64 entries (group g exists when count[g] != 0)."""
         cnt, sm, mn, mx, offs = self.results()
         lo, hi = int(offs[i]), int(offs[i + 1])
         n = hi - lo
@@ -122,7 +123,11 @@ class BatchEngine:
             if n:
                 call("qk_memcpy_dtoh", a.ctypes.data, p + lo * 8, n * 8, None)
             arrs.append(a)
-        c, s, lo_v, hi_v = arrs
+        return arrs
+
+    def result_host(self, i: int) -> dict:
+        """Groups of query i of the last run: key -> (sum, min, max, count)."""
+        c, s, lo_v, hi_v = self.result_arrays(i)
         keys = np.nonzero(c)[0]
         return {int(k): (int(s[k]), int(lo_v[k]), int(hi_v[k]), int(c[k])) for k in keys}
 

@@ -92,19 +92,24 @@ def build_datagen(verbose: bool = False) -> str:
 
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "groupby_driver.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "groupby_driver")
+DISPATCHER_SRC = os.path.join(ROOT, "tests", "cpp", "dispatcher_driver.cpp")
+DISPATCHER = os.path.join(ROOT, "tests", "cpp", "dispatcher_driver")
 
 
 def build_driver(verbose: bool = False) -> str | None:
-    """The C++ caller of the drop-in headers (include/qikkdb/) used by the parity tests."""
-    if not os.path.exists(DRIVER_SRC):
-        return None
-    tmp = DRIVER + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
-          "-L", PKG, "-lqikkdb_amd", "-Wl,-rpath,$ORIGIN/../../qikkdb_amd", "-o", tmp])
-    os.replace(tmp, DRIVER)
-    if verbose:
-        print(f"built {DRIVER}")
-    return DRIVER
+    """The C++ callers of the drop-in headers (include/qikkdb/) used by the parity tests."""
+    out = None
+    for src, exe in ((DRIVER_SRC, DRIVER), (DISPATCHER_SRC, DISPATCHER)):
+        if not os.path.exists(src):
+            continue
+        tmp = exe + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+              "-L", PKG, "-lqikkdb_amd", "-Wl,-rpath,$ORIGIN/../../qikkdb_amd", "-o", tmp])
+        os.replace(tmp, exe)
+        if verbose:
+            print(f"built {exe}")
+        out = exe
+    return out
 
 
 def build(force: bool = True, jobs: int = 4, verbose: bool = False) -> None:

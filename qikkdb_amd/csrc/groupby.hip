@@ -268,6 +268,10 @@ __device__ __forceinline__ bool slot_used(const GView& g, uint64_t s) {
     return s < g.cap && k[g.kw - 1] != 0ull;
 }
 
+__device__ __forceinline__ uint64_t emit_slot(const GView& g, uint64_t i) {
+    return i == 0 ? g.cap : (i <= g.cap ? i - 1 : i);
+}
+
 __device__ __forceinline__ void put_typed(void* out, int type, int64_t pos, int64_t iv, double dv,
                                           bool real) {
     switch (type) {
@@ -378,10 +382,17 @@ __device__ __forceinline__ void emit_aggs(const GView& g, const OutSpec& os, uin
     }
 }
 
+// Row order: the NULL-key group first (the reference keeps it in slot 0, and its callers read
+// "first key is NULL" from bit 0 of the key null mask, GPUGroupBy.cuh:727-733), then the hashed
+// slots in slot order, then the side slot of the key equal to the empty marker.  Position i of
+// that order is slot emit_slot(i).  Slots whose bit is set in `skip` (the seeded keys, emitted
+// by k_gb_emit_seeded) are left out; the other groups go to rows base, base + 1, ...
 __global__ void __launch_bounds__(kEmitBlock) k_gb_emit(GView g, KeySpec ks, OutSpec os,
                                                         uint64_t* status, uint32_t* ticket,
                                                         uint32_t epoch, int64_t tiles,
-                                                        int32_t* total, int64_t limit) {
+                                                        int32_t* total, int64_t limit,
+                                                        const uint64_t* __restrict__ skip,
+                                                        int64_t base) {
     constexpr int W = kEmitBlock / 64;
     __shared__ int64_t s_tile;
     __shared__ int32_t s_cnt[kEmitSteps][W];
@@ -395,8 +406,9 @@ __global__ void __launch_bounds__(kEmitBlock) k_gb_emit(GView g, KeySpec ks, Out
     uint32_t keep = 0, pre[kEmitSteps];
 #pragma unroll
     for (int k = 0; k < kEmitSteps; k++) {
-        const uint64_t s = (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid;
-        const bool sel = s < nslots && slot_used(g, s);
+        const uint64_t i = (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid;
+        const uint64_t s = emit_slot(g, i);
+        const bool sel = i < nslots && slot_used(g, s) && !(skip && ((skip[s >> 6] >> (s & 63)) & 1ull));
         const uint64_t b = __ballot(sel);
         pre[k] = (uint32_t)__popcll(b & lt_mask);
         if (lane == 0) s_cnt[k][wid] = (int32_t)__popcll(b);
@@ -414,12 +426,12 @@ __global__ void __launch_bounds__(kEmitBlock) k_gb_emit(GView g, KeySpec ks, Out
         }
     }
     __syncthreads();
-    int64_t step_base = s_excl;
+    int64_t step_base = base + (int64_t)s_excl;
     for (int k = 0; k < kEmitSteps; k++) {
         int64_t wave_base = step_base;
         for (int w = 0; w < wid; w++) wave_base += s_cnt[k][w];
         if ((keep >> k) & 1u) {
-            const uint64_t s = (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid;
+            const uint64_t s = emit_slot(g, (uint64_t)tile * kEmitTile + (uint64_t)k * kEmitBlock + tid);
             const int64_t pos = wave_base + pre[k];
             if (pos < limit) {  // outputs hold `limit` rows; the host checks the total
                 emit_key(g, ks, os, s, pos);
@@ -430,6 +442,85 @@ __global__ void __launch_bounds__(kEmitBlock) k_gb_emit(GView g, KeySpec ks, Out
         for (int w = 0; w < W; w++) st += s_cnt[k][w];
         step_base += st;
     }
+}
+
+// Row i of the results is seed key i (the pre-seeded table keeps its keys' order).
+__global__ void k_gb_emit_seeded(GView g, KeySpec ks, OutSpec os, const int64_t* __restrict__ seed_slot,
+                                 int32_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = (uint64_t)seed_slot[i];
+        emit_key(g, ks, os, s, i);
+        emit_aggs(g, os, s, i);
+    }
+}
+
+// Insert n keys of one key column (key 0 is the NULL group when first_null) as groups of
+// their own, whatever rows later reach them; slot_out[i] receives key i's slot and the slot's
+// bit is set in `bits`.
+__global__ void k_gb_seed(GView g, int32_t ktype, const void* __restrict__ keys, int32_t n,
+                          int32_t first_null, int64_t* slot_out, uint64_t* bits) {
+    __shared__ unsigned int s_new;
+    if (threadIdx.x == 0) s_new = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool kn = first_null && i == 0;
+        const uint64_t k = kn ? 0ull : key_single(ktype, keys, i);
+        bool is_new = false;
+        const int64_t s = g_slot1(g, k, kn, hash64(k), &is_new);
+        if (s < 0) {
+            g_set_error(g, QK_HASH_TABLE_FULL);
+        } else {
+            atomicOr((unsigned long long*)(bits + (s >> 6)), 1ull << (s & 63));
+        }
+        if (is_new) atomicAdd(&s_new, 1u);
+        slot_out[i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) g_count_new(g, s_new);
+}
+
+// Fold every group of table `src` into table `dst` (same key layout and aggregate states):
+// each group's key is inserted into dst and its states merged with atomics, so the result is
+// that of one table that saw both tables' rows.
+template <int KW>
+__global__ void __launch_bounds__(256) k_gb_merge(GView dst, GView src, uint32_t fields) {
+    __shared__ unsigned int s_new;
+    if (threadIdx.x == 0) s_new = 0;
+    __syncthreads();
+    const uint64_t nslots = src.cap + 2;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
+         s += (uint64_t)gridDim.x * blockDim.x) {
+        if (!slot_used(src, s)) continue;
+        Part p;
+        p.cnt = src.cnt[s];
+        p.cntall = src.cntall ? src.cntall[s] : 0ull;
+        p.sum = src.sum ? src.sum[s] : 0ull;
+        p.mn = src.mn ? src.mn[s] : ~0ull;
+        p.mx = src.mx ? src.mx[s] : 0ull;
+        bool is_new = false;
+        int64_t d;
+        if constexpr (KW == 1) {
+            if (s == src.cap) {
+                d = g_slot1(dst, 0ull, true, 0, &is_new);
+            } else {
+                const uint64_t k = s == src.cap + 1 ? kEmpty : src.keys[s];
+                d = g_slot1(dst, k, false, hash64(k), &is_new);
+            }
+        } else {
+            uint64_t k[KW];
+#pragma unroll
+            for (int w = 0; w < KW; w++) k[w] = src.keys[s * KW + w];
+            d = g_slotN<KW>(dst, k, key_hash<KW>(k), &is_new);
+        }
+        if (d < 0) {
+            g_set_error(dst, QK_HASH_TABLE_FULL);
+            continue;
+        }
+        if (is_new) atomicAdd(&s_new, 1u);
+        g_accum_atomic(dst, d, p, fields);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) g_count_new(dst, s_new);
 }
 
 // ---- launch helpers -------------------------------------------------------------------------
@@ -520,6 +611,11 @@ struct qk_groupby_s {
     int32_t wgs_per_cu = 0;   // 0 = as many as fit
     int32_t last_H = 0;
     int32_t last_path = 0;
+    // pre-seeded keys (GPUGroupBy(maxHashCount, keys, firstKeyIsNull)): their slots in seed
+    // order and a bit per slot; GetResults emits them first, in that order
+    int64_t* seed_slot = nullptr;
+    uint64_t* seed_bits = nullptr;
+    int32_t seed_n = 0;
     // profiling
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -568,6 +664,14 @@ int init_table(qk_groupby_s* h, hipStream_t s) {
     k_gb_init<<<(unsigned)grid, 256, 0, s>>>(g, h->fields);
     QK_HIP(hipGetLastError());
     QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
+    if (h->seed_slot || h->seed_bits) {  // a reset drops the seeded keys too
+        QK_HIP(hipStreamSynchronize(s));
+        if (h->seed_slot) pool_free(h->seed_slot);
+        if (h->seed_bits) pool_free(h->seed_bits);
+        h->seed_slot = nullptr;
+        h->seed_bits = nullptr;
+        h->seed_n = 0;
+    }
     h->tp.heavy_ready = false;
     h->tp.pending_rows = 0;  // rows of the previous query still in the regions are dropped
     return QK_SUCCESS;
@@ -708,7 +812,7 @@ int collect_profile(qk_groupby h) {
 }
 
 void free_table(qk_groupby_s* h) {
-    void* ptrs[] = {h->keys, h->cnt, h->cntall, h->sum, h->mn, h->mx, h->ctr};
+    void* ptrs[] = {h->keys, h->cnt, h->cntall, h->sum, h->mn, h->mx, h->ctr, h->seed_slot, h->seed_bits};
     for (void* p : ptrs)
         if (p) pool_free(p);
     twophase_free(h->tp);
@@ -936,8 +1040,16 @@ int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream) {
     return QK_SUCCESS;
 }
 
-int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, void** out_values,
-                      uint64_t** out_value_nulls, int32_t* out_count, void* stream) {
+}  // extern "C"
+
+namespace qk {
+namespace {
+
+// GetResults with the outputs `aggs` (naggs of them, output types otypes) computed from the
+// table's aggregate states.
+int get_results_impl(qk_groupby h, int32_t naggs, const int32_t* aggs, const int32_t* otypes,
+                     void** out_keys, uint64_t** out_key_nulls, void** out_values,
+                     uint64_t** out_value_nulls, int32_t* out_count, void* stream) {
     if (!h || !out_count) return fail(QK_ERROR, "qk_gb_get_results: null argument");
     if (!out_keys) return fail(QK_ERROR, "qk_gb_get_results: null key output array");
     QK_TRY(ensure_init());
@@ -947,7 +1059,7 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
         out_keys[c] = nullptr;
         if (out_key_nulls) out_key_nulls[c] = nullptr;
     }
-    for (int a = 0; a < h->naggs; a++) {
+    for (int a = 0; a < naggs; a++) {
         if (out_values) out_values[a] = nullptr;
         if (out_value_nulls) out_value_nulls[a] = nullptr;
     }
@@ -959,7 +1071,7 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
     if (groups > INT32_MAX) return fail(QK_ERROR, "qk_gb_get_results: too many groups");
 
     OutSpec os{};
-    os.naggs = h->naggs;
+    os.naggs = naggs;
     os.vtype = h->value_type;
     std::vector<void*> owned;
     auto cleanup = [&]() {
@@ -979,11 +1091,11 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
             if (st == QK_SUCCESS) st = hipMemsetAsync(os.konull[c], 0, nwords * 8, s) == hipSuccess ? QK_SUCCESS : QK_RUNTIME_ERROR;
         }
     }
-    for (int a = 0; a < h->naggs && st == QK_SUCCESS; a++) {
-        os.agg[a] = h->aggs[a];
-        os.otype[a] = h->out_types[a];
-        if (h->aggs[a] == QK_AGG_NONE || !out_values) continue;
-        st = alloc((size_t)groups * type_size(h->out_types[a]), &os.out[a]);
+    for (int a = 0; a < naggs && st == QK_SUCCESS; a++) {
+        os.agg[a] = aggs[a];
+        os.otype[a] = otypes[a];
+        if (aggs[a] == QK_AGG_NONE || !out_values) continue;
+        st = alloc((size_t)groups * type_size(otypes[a]), &os.out[a]);
         if (st == QK_SUCCESS && out_value_nulls) {
             st = alloc(nwords * 8, (void**)&os.onull[a]);
             if (st == QK_SUCCESS) st = hipMemsetAsync(os.onull[a], 0, nwords * 8, s) == hipSuccess ? QK_SUCCESS : QK_RUNTIME_ERROR;
@@ -1007,8 +1119,13 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
     GView g = view_of(h);
     k_gb_emit<<<(unsigned)tiles, kEmitBlock, 0, s>>>(g, h->ks, os, words + kLbHeaderWords,
                                                      (uint32_t*)words, epoch, tiles, total,
-                                                     groups);
+                                                     groups, h->seed_bits, h->seed_n);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess && h->seed_n > 0) {
+        const int64_t grid = std::min<int64_t>((h->seed_n + 255) / 256, (int64_t)device_info().cus * 8);
+        k_gb_emit_seeded<<<(unsigned)grid, 256, 0, s>>>(g, h->ks, os, h->seed_slot, h->seed_n);
+        e = hipGetLastError();
+    }
     int32_t written = 0;
     if (e == hipSuccess) {
         void* hb = nullptr;
@@ -1023,6 +1140,7 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
         cleanup();
         return st != QK_SUCCESS ? st : fail(QK_RUNTIME_ERROR, "qk_gb_get_results: %s", hipGetErrorString(e));
     }
+    written += h->seed_n;
     if (written != (int32_t)groups) {
         cleanup();
         return fail(QK_ERROR, "qk_gb_get_results: %d groups emitted, %lld counted", written, (long long)groups);
@@ -1031,12 +1149,115 @@ int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, v
         out_keys[c] = os.kout[c];
         if (out_key_nulls) out_key_nulls[c] = os.konull[c];
     }
-    for (int a = 0; a < h->naggs; a++) {
+    for (int a = 0; a < naggs; a++) {
         if (out_values) out_values[a] = os.out[a];
         if (out_value_nulls) out_value_nulls[a] = os.onull[a];
     }
     *out_count = (int32_t)groups;
     return QK_SUCCESS;
+}
+
+}  // namespace
+}  // namespace qk
+
+extern "C" {
+
+int qk_gb_get_results(qk_groupby h, void** out_keys, uint64_t** out_key_nulls, void** out_values,
+                      uint64_t** out_value_nulls, int32_t* out_count, void* stream) {
+    if (!h) return fail(QK_ERROR, "qk_gb_get_results: null table");
+    return get_results_impl(h, h->naggs, h->aggs, h->out_types, out_keys, out_key_nulls, out_values,
+                            out_value_nulls, out_count, stream);
+}
+
+int qk_gb_get_results_aggs(qk_groupby h, int32_t naggs, const int32_t* aggs, const int32_t* out_types,
+                           void** out_keys, uint64_t** out_key_nulls, void** out_values,
+                           uint64_t** out_value_nulls, int32_t* out_count, void* stream) {
+    if (!h) return fail(QK_ERROR, "qk_gb_get_results_aggs: null table");
+    if (naggs < 1 || naggs > kMaxAggs || !aggs) return fail(QK_ERROR, "qk_gb_get_results_aggs: 1..%d aggregates", kMaxAggs);
+    int32_t ot[kMaxAggs];
+    for (int a = 0; a < naggs; a++) {
+        const int ag = aggs[a];
+        uint32_t need = 0;
+        if (ag == QK_AGG_SUM || ag == QK_AGG_AVG) need = F_SUM;
+        else if (ag == QK_AGG_MIN) need = F_MIN;
+        else if (ag == QK_AGG_MAX) need = F_MAX;
+        else if (ag == QK_AGG_COUNT_ALL) need = F_CNTALL;
+        else if (ag != QK_AGG_COUNT && ag != QK_AGG_NONE) return fail(QK_UNKNOWN_AGG_FUN, "unknown aggregate %d", ag);
+        if ((h->fields & need) != need)
+            return fail(QK_ERROR, "qk_gb_get_results_aggs: aggregate %d needs a state the table does not keep", ag);
+        const int dflt = (ag == QK_AGG_COUNT || ag == QK_AGG_COUNT_ALL) ? QK_LONG : h->value_type;
+        ot[a] = (out_types && out_types[a] > 0) ? out_types[a] : dflt;
+        if ((ag == QK_AGG_COUNT || ag == QK_AGG_COUNT_ALL) && ot[a] != QK_LONG)
+            return fail(QK_ERROR, "Output value data type in GROUP BY with COUNT must be int64_t");
+        if (!type_size(ot[a])) return fail(QK_ERROR, "qk_gb_get_results_aggs: output type %d unsupported", ot[a]);
+    }
+    return get_results_impl(h, naggs, aggs, ot, out_keys, out_key_nulls, out_values, out_value_nulls, out_count,
+                            stream);
+}
+
+int qk_gb_seed_keys(qk_groupby h, const void* keys, int32_t n, int32_t first_key_is_null, void* stream) {
+    if (!h) return fail(QK_ERROR, "qk_gb_seed_keys: null table");
+    QK_TRY(ensure_init());
+    if (h->nkeys != 1) return fail(QK_ERROR, "qk_gb_seed_keys: single-key tables only");
+    if (n < 0 || (n > 0 && !keys)) return fail(QK_ERROR, "qk_gb_seed_keys: bad key buffer");
+    if (h->seed_n > 0) return fail(QK_ERROR, "qk_gb_seed_keys: the table is already seeded");
+    if (n == 0) return QK_SUCCESS;
+    hipStream_t s = as_stream(stream);
+    QK_TRY(flush_pending(h, s));
+    TableCounters before;
+    QK_TRY(read_counters(h, s, &before));
+    if (before.nkeys != 0) return fail(QK_ERROR, "qk_gb_seed_keys: the table already holds groups");
+    const size_t nbits = (size_t)((h->cap + 2 + 63) / 64);
+    QK_TRY(pool_alloc((void**)&h->seed_slot, (size_t)n * 8));
+    int st = pool_alloc((void**)&h->seed_bits, nbits * 8);
+    if (st != QK_SUCCESS) {
+        pool_free(h->seed_slot);
+        h->seed_slot = nullptr;
+        return st;
+    }
+    QK_HIP(hipMemsetAsync(h->seed_bits, 0, nbits * 8, s));
+    const int64_t grid = std::min<int64_t>((n + 255) / 256, (int64_t)device_info().cus * 8);
+    k_gb_seed<<<(unsigned)grid, 256, 0, s>>>(view_of(h), h->key_types[0], keys, n, first_key_is_null ? 1 : 0,
+                                             h->seed_slot, h->seed_bits);
+    QK_HIP(hipGetLastError());
+    h->seed_n = n;
+    st = take_error(h, s);
+    TableCounters after;
+    if (st == QK_SUCCESS) st = read_counters(h, s, &after);
+    const uint64_t want = (uint64_t)n - (first_key_is_null ? 1 : 0);
+    if (st == QK_SUCCESS && after.nkeys != want)
+        st = fail(QK_ERROR, "qk_gb_seed_keys: %llu distinct keys among %llu (the seed keys must be distinct)",
+                  (unsigned long long)after.nkeys, (unsigned long long)want);
+    if (st != QK_SUCCESS) init_table(h, s);  // back to an empty table
+    return st;
+}
+
+int qk_gb_merge(qk_groupby dst, qk_groupby src, void* stream) {
+    if (!dst || !src) return fail(QK_ERROR, "qk_gb_merge: null table");
+    if (dst == src) return fail(QK_ERROR, "qk_gb_merge: a table cannot be merged into itself");
+    QK_TRY(ensure_init());
+    bool same = dst->nkeys == src->nkeys && dst->value_type == src->value_type && dst->fields == src->fields &&
+                dst->ks.kw == src->ks.kw && dst->ks.nullable == src->ks.nullable && dst->real_sum == src->real_sum;
+    for (int c = 0; same && c < dst->nkeys; c++) same = dst->key_types[c] == src->key_types[c];
+    if (!same) return fail(QK_ERROR, "qk_gb_merge: the tables differ in key or aggregate layout");
+    int cur = -1;
+    QK_HIP(hipGetDevice(&cur));
+    if (dst->device != cur || src->device != cur)
+        return fail(QK_ERROR, "qk_gb_merge: both tables must live on the calling thread's device (device %d "
+                    "and %d, current %d); a merge across GPUs is not built", dst->device, src->device, cur);
+    hipStream_t s = as_stream(stream);
+    QK_TRY(flush_pending(src, s));
+    QK_TRY(flush_pending(dst, s));
+    QK_TRY(take_error(src, s));
+    GView gd = view_of(dst), gs = view_of(src);
+    const int64_t grid = std::min<int64_t>((int64_t)((src->cap + 2 + 255) / 256), (int64_t)device_info().cus * 16);
+    switch (dst->ks.kw) {
+    case 1: k_gb_merge<1><<<(unsigned)grid, 256, 0, s>>>(gd, gs, dst->fields); break;
+    case 2: k_gb_merge<2><<<(unsigned)grid, 256, 0, s>>>(gd, gs, dst->fields); break;
+    default: k_gb_merge<4><<<(unsigned)grid, 256, 0, s>>>(gd, gs, dst->fields); break;
+    }
+    QK_HIP(hipGetLastError());
+    return take_error(dst, s);
 }
 
 int qk_gb_set_profiling(qk_groupby h, int32_t enable) {

@@ -1379,6 +1379,10 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     const DeviceInfo& d = device_info();
     const int vbytes = has_values ? type_size(value_type) : 0;
     const int64_t rows = region_rows_for(b, n, defer);
+    // the old buffers go back to the shared pool, where another stream or thread may be handed
+    // them at once: wait for every kernel that may still read them (a deferred phase 2, on
+    // whatever stream the table was last used from) before giving them up
+    if (b.r_rec) QK_HIP(hipDeviceSynchronize());
     twophase_free(b);
     b.W = d.cus > 0 ? std::min(d.cus, 1024) : 256;
     b.P = parts;

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from helpers import as_map
 from qikkdb_amd import build as B
 from qikkdb_amd import datagen
 
@@ -55,3 +56,63 @@ def test_driver_ragged_blocks_and_empty_block(tmp_path):
     vals[:30_000] = -5  # the first block keeps no row
     got = run_driver(tmp_path, keys, vals, 30_000, 10)
     assert got == oracle(keys, vals, 10)
+
+
+# ---- the dispatcher-shaped caller: GroupByHelper over groupByTables_ ---------------------------
+
+AGG_CODES = {"min": O.AGG_MIN, "max": O.AGG_MAX, "sum": O.AGG_SUM, "avg": O.AGG_AVG, "count": O.AGG_COUNT}
+
+
+def _pack(isnull):
+    from helpers import pack_bits
+    return pack_bits(isnull)
+
+
+def test_dispatcher_shaped_caller(tmp_path):
+    """GroupByHelper-shaped single- and composite-key queries over two dispatcher tables merged by
+    GetResults(..., groupByTables_, ...), each aggregate against the oracle; the driver's own
+    checks of reconstructCol, GenerateIndexes, PrefixSum, Sum/SumKeep, CompressNullMask, the
+    pre-seeded constructor and ReconstructRawNumbers must all pass."""
+    assert os.path.exists(B.DISPATCHER), "tests/cpp/dispatcher_driver is built by qikkdb_amd.build"
+    rng = np.random.default_rng(12)
+    n = 60_001
+    a = rng.integers(-40, 40, n).astype(np.int32)
+    b = rng.integers(0, 6, n).astype(np.int32)
+    v = rng.integers(-1000, 1000, n).astype(np.int32)
+    f = rng.integers(-100, 100, n).astype(np.int32)
+    an, bn, vn = rng.random(n) < 0.03, rng.random(n) < 0.03, rng.random(n) < 0.1
+    for name, arr in (("a", a), ("b", b), ("v", v), ("f", f)):
+        arr.tofile(tmp_path / f"{name}.i32")
+    for name, arr in (("an", an), ("bn", bn), ("vn", vn)):
+        arr.astype(np.uint8).tofile(tmp_path / f"{name}.u8")
+    threshold, block = -30, 7_000
+    r = subprocess.run([B.DISPATCHER, str(tmp_path), str(n), str(block), str(threshold)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    checks = set()
+    for line in r.stdout.splitlines():
+        p = line.split()
+        if p[0] == "CHECK":
+            checks.add(p[1])
+            continue
+        agg = p[1]
+        if p[0] == "S":
+            key = (None if int(p[2]) else int(p[3]),)
+            val = None if int(p[4]) else int(p[5])
+        else:
+            key = (None if int(p[2]) else int(p[3]), None if int(p[4]) else int(p[5]))
+            val = None if int(p[6]) else int(p[7])
+        m = got.setdefault((p[0], agg), {})
+        assert key not in m, f"duplicate group {key} in {p[0]} {agg}"
+        m[key] = val
+    assert checks == {"reconstructCol", "GenerateIndexes", "PrefixSum", "Sum", "CompressNullMask", "seeded",
+                      "ReconstructRawNumbers", "empty"}
+    preds = [O.pred(0, f, threshold, O.INT)]
+    for form, keys, kn in (("S", [a], [_pack(an)]), ("M", [a, b], [_pack(an), _pack(bn)])):
+        for agg, code in AGG_CODES.items():
+            out_types = [O.LONG] if code == O.AGG_COUNT else None
+            g = O.GroupBy([code], [O.INT] * len(keys), O.INT, out_types)
+            g.process(keys, v, _pack(vn), kn, preds=preds)
+            want = as_map(g.results())
+            assert got[(form, agg)] == want, f"{form} {agg} differs from the oracle"

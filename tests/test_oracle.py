@@ -264,3 +264,34 @@ def test_threaded_equals_single():
     sel = (vals >= np.float32(0.5)) & (vals <= np.float32(4.0)) & ~unpack_bits(vnull, n)
     assert sum(as_map(g1.results(), 1).values()) == int(sel.sum())
     assert abs(sum(as_map(g1.results(), 0).values()) - float(vals[sel].astype(np.float64).sum())) < 1e-6 * float(vals[sel].sum())
+
+
+def test_compare_groups_catches_planted_differences():
+    """The all-groups checker used at full benchmark size: equal results pass whatever their
+    order; a changed value, a changed key, a flipped NULL flag or an FP error above the
+    tolerance each fail."""
+    import copy
+    rng = np.random.default_rng(3)
+    n = 50_000
+    keys = rng.integers(0, 3000, n).astype(np.int64)
+    vals = rng.lognormal(0, 1, n).astype(np.float32)
+    vn = rng.random(n) < 0.3
+    g = O.GroupBy([O.AGG_SUM, O.AGG_COUNT, O.AGG_MAX], O.LONG, O.FLOAT)
+    g.process([keys], vals, pack_bits(vn))
+    a = g.results()
+    perm = rng.permutation(a["count"])
+    b = {"count": a["count"], "keys": [a["keys"][0][perm]], "key_nulls": [a["key_nulls"][0][perm]],
+         "values": [v[perm] for v in a["values"]], "value_nulls": [v[perm] for v in a["value_nulls"]]}
+    O.compare_groups(b, a, rel_aggs=(0,), rel=1e-5)
+    for plant in ("value", "key", "null", "fp"):
+        c = copy.deepcopy(b)
+        if plant == "value":
+            c["values"][1][7] += 1
+        elif plant == "key":
+            c["keys"][0][11] = 10_000
+        elif plant == "null":
+            c["value_nulls"][2][5] = ~c["value_nulls"][2][5]
+        else:
+            c["values"][0][3] *= 1 + 3e-5
+        with pytest.raises(AssertionError):
+            O.compare_groups(c, a, rel_aggs=(0,), rel=1e-5)
