@@ -198,8 +198,14 @@ enum qk_gb_flags {
     /* ProcessBlock calls do not wait for the kernel; a HASH_TABLE_FULL latched by one of them
      * is returned by qk_gb_check_error / qk_gb_get_results instead. */
     QK_GB_ASYNC = 1u << 1,
-    /* Instead of failing with QK_HASH_TABLE_FULL the table grows itself (rows that found no
-     * slot are re-aggregated after the growth; nothing restarts). */
+    /* Instead of failing with QK_HASH_TABLE_FULL the table grows itself: after each block it
+     * checks itself (a host wait, so such a table neither returns early nor defers its merge),
+     * and past its key capacity it moves its groups into a table of at least twice the
+     * capacity.  Rows whose keys found no slot are aggregated again into the grown table: they
+     * are the rows whose key it lacks, so no block restarts and no row counts twice.  The
+     * reference grows and restarts the query instead (GpuSqlDispatcher.cpp:378-394).  The
+     * largest table holds 2^30 keys; past that QK_HASH_TABLE_FULL is returned.  Keys cannot be
+     * pre-seeded into such a table (qk_gb_seed_keys). */
     QK_GB_AUTO_GROW = 1u << 2
 };
 
@@ -218,7 +224,8 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
 int qk_gb_destroy(qk_groupby h);
 /* Drop every group; the table is then as freshly created. */
 int qk_gb_reset(qk_groupby h, void* stream);
-/* GPUGroupBy::GetMaxHashCount (GPUGroupBy.cuh:414-417): the key capacity. */
+/* GPUGroupBy::GetMaxHashCount (GPUGroupBy.cuh:414-417): the key capacity (of a growing table:
+ * its current capacity). */
 int qk_gb_get_max_hash_count(qk_groupby h, int32_t* out);
 
 /* Aggregate one block of n rows.  keys[nkeys] are device columns; key_nulls is NULL or an
@@ -326,8 +333,8 @@ int qk_gb_set_defer_rows(qk_groupby h, int64_t rows);
  * out[0] path of the last block (0 one-pass, 1 two-phase), out[1] heavy keys, out[2] heavy
  * count threshold, out[3] rows written to partition regions by the last block, out[4]
  * phase-1 workgroups, out[5] partitions, out[6] rows per region, out[7] phase-1 LDS slots,
- * out[8] phase-2 LDS slots, out[9] heavy key limit, out[10] sub-tables of the table.
- * n = entries to fill (at most 11). */
+ * out[8] phase-2 LDS slots, out[9] heavy key limit, out[10] sub-tables of the table, out[11]
+ * growths of a QK_GB_AUTO_GROW table.  n = entries to fill (at most 12). */
 int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
 
 /* ======================================================================================

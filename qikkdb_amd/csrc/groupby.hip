@@ -31,6 +31,7 @@ namespace qk {
 namespace {
 
 constexpr int kSteps = 4;        // rows per thread per tile
+constexpr uint64_t kMaxLogical = (uint64_t)1 << 30;  // GB_BUFFER_SIZE_MAX (IGroupBy.h:6)
 constexpr int kLdsProbes = 8;    // LDS probe limit before a row goes to the global table
 constexpr int kEmitBlock = 256;
 constexpr int kEmitSteps = 16;
@@ -198,11 +199,11 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
                     lds_accum<REAL>(t, ls, p, fields);
                 } else {
                     const int64_t gs = g_slot1(g, key[k][0], kn[k], h, &is_new);
-                    if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+                    if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_lost(g);
                 }
             } else {
                 const int64_t gs = g_slotN<KW>(g, key[k], h, &is_new);
-                if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+                if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_lost(g);
             }
             if (is_new) atomicAdd(&s_new, 1u);
         }
@@ -229,7 +230,7 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
                 gs = g_slot1(g, k, false, hash64(k), &is_new);
             }
             if (is_new) atomicAdd(&s_new, 1u);
-            if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+            if (gs >= 0) g_accum_atomic(g, gs, p, fields); else g_set_lost(g);
         }
     }
     __syncthreads();
@@ -513,7 +514,7 @@ __global__ void __launch_bounds__(256) k_gb_merge(GView dst, GView src, uint32_t
             d = g_slotN<KW>(dst, k, key_hash<KW>(k), &is_new);
         }
         if (d < 0) {
-            g_set_error(dst, QK_HASH_TABLE_FULL);
+            g_set_lost(dst);
             continue;
         }
         if (is_new) atomicAdd(&s_new, 1u);
@@ -521,6 +522,34 @@ __global__ void __launch_bounds__(256) k_gb_merge(GView dst, GView src, uint32_t
     }
     __syncthreads();
     if (threadIdx.x == 0) g_count_new(dst, s_new);
+}
+
+// Rows of a block still to aggregate after the table dropped partials (a growing table): rows
+// that pass the block's WHERE (MODE 0 none, 1 int8 mask, 2 predicate list) and whose key the
+// table does not hold.  out[r] = 1 for those rows, 0 otherwise.  `out` may be `mask` itself
+// (each thread reads its row's mask entry before writing it).
+template <int KW, int MODE>
+__global__ void __launch_bounds__(256) k_gb_absent(GView g, KeySpec ks, KeyCols kc,
+                                                   const int8_t* mask, PredSet preds, int64_t n,
+                                                   int8_t* out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        bool pass = true;
+        if (MODE == 1) pass = mask[r] != 0;
+        if (MODE == 2)
+            for (int p = 0; p < preds.n && pass; p++) pass = eval_pred(preds.p[p], r);
+        bool absent = false;
+        if (pass) {
+            uint64_t k[KW];
+            bool kn = false;
+            load_key<KW>(ks, kc, r, k, kn);
+            if constexpr (KW == 1)
+                absent = !g_has1(g, k[0], kn, hash64(k[0]));
+            else
+                absent = !g_hasN<KW>(g, k, key_hash<KW>(k));
+        }
+        out[r] = (int8_t)(pass && absent);
+    }
 }
 
 // ---- launch helpers -------------------------------------------------------------------------
@@ -611,6 +640,7 @@ struct qk_groupby_s {
     int32_t wgs_per_cu = 0;   // 0 = as many as fit
     int32_t last_H = 0;
     int32_t last_path = 0;
+    int64_t grows = 0;      // growths of a QK_GB_AUTO_GROW table
     // pre-seeded keys (GPUGroupBy(maxHashCount, keys, firstKeyIsNull)): their slots in seed
     // order and a bit per slot; GetResults emits them first, in that order
     int64_t* seed_slot = nullptr;
@@ -648,7 +678,61 @@ GView view_of(const qk_groupby_s* h) {
     g.nkeys = &h->ctr->nkeys;
     g.err = &h->ctr->err;
     g.logical_cap = h->logical_cap;
+    g.lost = &h->ctr->lost;
     return g;
+}
+
+// Slot count and sub-table layout of a table holding `logical` keys (before it reports itself
+// full): 2 slots per key, rounded up to a power of two.
+void layout_table(qk_groupby_s* h, uint64_t logical) {
+    h->logical_cap = logical;
+    h->reported_max = (int32_t)(h->nkeys == 1 ? logical + 1 : logical);  // + 1 for the NULL key
+    uint64_t cap = 64;
+    while (cap < 2 * logical) cap <<= 1;
+    h->cap = cap;
+    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only); more
+    // of them when that lets a sub-table fit the LDS, so that the two-phase path folds
+    // straight into its sub-tables.  Cold rows go to at most kMaxRecParts partitions, each
+    // holding the rows of up to 2^kMaxSplitBits sub-tables.
+    int32_t parts = 1;
+    if (h->ks.kw == 1) {
+        while (parts < 256 && cap / (uint64_t)(parts * 2) >= 1024) parts *= 2;
+        const size_t budget = std::min<size_t>(device_info().lds_per_block, 160 * 1024);
+        int32_t more = parts;
+        while (more < (kMaxRecParts << kMaxSplitBits) &&
+               p2o_lds_bytes((int64_t)(cap / (uint64_t)more), h->fields) > budget &&
+               cap / (uint64_t)(more * 2) >= 1024)
+            more *= 2;
+        if (p2o_lds_bytes((int64_t)(cap / (uint64_t)more), h->fields) <= budget) parts = more;
+    }
+    h->parts = parts;
+    h->rec_parts = std::min<int32_t>(parts, kMaxRecParts);
+}
+
+// Slot arrays for h->cap (+ 2 side slots); on failure every array is released.
+int alloc_slots(qk_groupby_s* h) {
+    const size_t slots = (size_t)(h->cap + 2);
+    const uint32_t fields = h->fields;
+    int st = QK_SUCCESS;
+    auto A = [&](unsigned long long** p, size_t bytes) {
+        if (st == QK_SUCCESS) st = pool_alloc((void**)p, bytes);
+    };
+    h->keys = nullptr;
+    h->cnt = h->cntall = h->sum = h->mn = h->mx = nullptr;
+    A((unsigned long long**)&h->keys, slots * h->ks.kw * 8);
+    A(&h->cnt, slots * 8);
+    if (fields & F_CNTALL) A(&h->cntall, slots * 8);
+    if (fields & F_SUM) A(&h->sum, slots * 8);
+    if (fields & F_MIN) A(&h->mn, slots * 8);
+    if (fields & F_MAX) A(&h->mx, slots * 8);
+    if (st != QK_SUCCESS) {
+        void* ptrs[] = {h->keys, h->cnt, h->cntall, h->sum, h->mn, h->mx};
+        for (void* q : ptrs)
+            if (q) pool_free(q);
+        h->keys = nullptr;
+        h->cnt = h->cntall = h->sum = h->mn = h->mx = nullptr;
+    }
+    return st;
 }
 
 bool agg_needs_value(int a) {
@@ -706,6 +790,66 @@ int take_error(qk_groupby_s* h, hipStream_t s) {
     return QK_SUCCESS;
 }
 
+// Grow the table until it holds at least `need` keys (at least doubling it): new slot arrays,
+// every group moved over by the merge kernel (a key keeps its states), the old arrays freed.
+// Deferred rows are merged into the old table first.  Waits for the table's work.
+int grow_table(qk_groupby_s* h, uint64_t need, hipStream_t s) {
+    if (h->logical_cap >= kMaxLogical)
+        return fail(QK_HASH_TABLE_FULL, "GROUP BY hash table is full at its largest size (%llu keys)",
+                    (unsigned long long)h->logical_cap);
+    QK_TRY(flush_pending(h, s));
+    const GView old = view_of(h);
+    qk_groupby_s keep = {};
+    keep.keys = h->keys;
+    keep.cnt = h->cnt;
+    keep.cntall = h->cntall;
+    keep.sum = h->sum;
+    keep.mn = h->mn;
+    keep.mx = h->mx;
+    const uint64_t old_logical = h->logical_cap;
+    const uint64_t old_cap = h->cap;
+    const int32_t old_parts = h->parts, old_rec = h->rec_parts, old_max = h->reported_max;
+    uint64_t logical = old_logical * 2;
+    while (logical < need && logical < kMaxLogical) logical *= 2;
+    layout_table(h, std::min<uint64_t>(logical, kMaxLogical));
+    int st = alloc_slots(h);
+    if (st != QK_SUCCESS) {  // the table stays as it was
+        h->keys = keep.keys;
+        h->cnt = keep.cnt;
+        h->cntall = keep.cntall;
+        h->sum = keep.sum;
+        h->mn = keep.mn;
+        h->mx = keep.mx;
+        h->logical_cap = old_logical;
+        h->cap = old_cap;
+        h->parts = old_parts;
+        h->rec_parts = old_rec;
+        h->reported_max = old_max;
+        return st;
+    }
+    GView g = view_of(h);
+    const uint64_t total = h->cap + 2;
+    const unsigned igrid = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)device_info().cus * 16);
+    k_gb_init<<<igrid, 256, 0, s>>>(g, h->fields);
+    QK_HIP(hipGetLastError());
+    QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
+    const unsigned mgrid = (unsigned)std::min<uint64_t>((old.cap + 2 + 255) / 256, (uint64_t)device_info().cus * 16);
+    switch (h->ks.kw) {
+    case 1: k_gb_merge<1><<<mgrid, 256, 0, s>>>(g, old, h->fields); break;
+    case 2: k_gb_merge<2><<<mgrid, 256, 0, s>>>(g, old, h->fields); break;
+    default: k_gb_merge<4><<<mgrid, 256, 0, s>>>(g, old, h->fields); break;
+    }
+    QK_HIP(hipGetLastError());
+    QK_HIP(hipStreamSynchronize(s));
+    void* ptrs[] = {keep.keys, keep.cnt, keep.cntall, keep.sum, keep.mn, keep.mx};
+    for (void* q : ptrs)
+        if (q) pool_free(q);
+    h->grows++;
+    // the two-phase buffers are laid out for the partition count: the next block resizes them
+    // (and samples its heavy keys again) when that changed
+    return QK_SUCCESS;
+}
+
 int choose_lds_slots(const qk_groupby_s* h) {
     if (h->ks.kw != 1 || h->lds_budget == 0) return 0;
     const DeviceInfo& d = device_info();
@@ -729,9 +873,9 @@ std::pair<hipEvent_t, hipEvent_t>* next_events(qk_groupby h) {
     return &h->events[h->events_used++];
 }
 
-int process_common(qk_groupby h, int mode, const void* const* keys, const uint64_t* const* key_nulls,
-                   const void* values, const uint64_t* value_nulls, const int8_t* mask,
-                   int32_t npreds, const qk_predicate* preds, int32_t n, void* stream) {
+int process_once(qk_groupby h, int mode, const void* const* keys, const uint64_t* const* key_nulls,
+                 const void* values, const uint64_t* value_nulls, const int8_t* mask,
+                 int32_t npreds, const qk_predicate* preds, int32_t n, void* stream) {
     if (!h) return fail(QK_ERROR, "group by: null table");
     QK_TRY(ensure_init());
     if (n <= 0) return QK_SUCCESS;  // GPUGroupBy.cuh:401
@@ -767,8 +911,9 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
     int st;
     if (two) {
         // Phase 2 is deferred across blocks only for asynchronous tables: a synchronous table
-        // reports a full table at the block that filled it, as the reference does.
-        const bool defer = (h->flags & QK_GB_ASYNC) != 0 && h->tp.defer_rows != 0;
+        // reports a full table at the block that filled it, as the reference does, and a
+        // growing table grows at the block that filled it.
+        const bool defer = (h->flags & (QK_GB_ASYNC | QK_GB_AUTO_GROW)) == QK_GB_ASYNC && h->tp.defer_rows != 0;
         TwoPhaseBufs& tp = h->tp;
         if (tp.pending_rows > 0 &&
             (twophase_needs_realloc(tp, h->rec_parts, h->value_type, values != nullptr, n, defer) ||
@@ -795,8 +940,81 @@ int process_common(qk_groupby h, int mode, const void* const* keys, const uint64
         if (st == QK_SUCCESS && ev) QK_HIP(hipEventRecord(ev->second, s));
     }
     if (st != QK_SUCCESS) return st;
-    if (!(h->flags & QK_GB_ASYNC)) return take_error(h, s);
+    if (!(h->flags & (QK_GB_ASYNC | QK_GB_AUTO_GROW))) return take_error(h, s);
     return QK_SUCCESS;
+}
+
+// Rows of the block not yet aggregated by a table that dropped partials: an int8 mask of the
+// rows that pass the block's WHERE and whose key the table lacks.
+int absent_rows(qk_groupby h, int mode, const KeyCols& kc, const int8_t* mask, const PredSet& ps,
+                int32_t n, int8_t* out, hipStream_t s) {
+    const GView g = view_of(h);
+    const unsigned grid = (unsigned)std::min<int64_t>(((int64_t)n + 255) / 256, (int64_t)device_info().cus * 16);
+#define QK_ABSENT(KW, M) k_gb_absent<KW, M><<<grid, 256, 0, s>>>(g, h->ks, kc, mask, ps, n, out)
+    switch (h->ks.kw * 4 + mode) {
+    case 4: QK_ABSENT(1, 0); break;
+    case 5: QK_ABSENT(1, 1); break;
+    case 6: QK_ABSENT(1, 2); break;
+    case 8: QK_ABSENT(2, 0); break;
+    case 9: QK_ABSENT(2, 1); break;
+    case 10: QK_ABSENT(2, 2); break;
+    case 16: QK_ABSENT(4, 0); break;
+    case 17: QK_ABSENT(4, 1); break;
+    default: QK_ABSENT(4, 2); break;
+    }
+#undef QK_ABSENT
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+// ProcessBlock.  A QK_GB_AUTO_GROW table checks itself after the block: past its key capacity it
+// grows (GpuSqlDispatcher.cpp:378-394 grows and restarts the query instead), and when the block
+// found sub-tables full, the rows whose keys did not get in are aggregated again into the grown
+// table.  Those are exactly the rows whose key the table lacks (g_set_lost), so no row is
+// counted twice and none is missed.
+int process_common(qk_groupby h, int mode, const void* const* keys, const uint64_t* const* key_nulls,
+                   const void* values, const uint64_t* value_nulls, const int8_t* mask,
+                   int32_t npreds, const qk_predicate* preds, int32_t n, void* stream) {
+    int st = process_once(h, mode, keys, key_nulls, values, value_nulls, mask, npreds, preds, n, stream);
+    if (st != QK_SUCCESS || !(h->flags & QK_GB_AUTO_GROW) || n <= 0) return st;
+    hipStream_t s = as_stream(stream);
+    KeyCols kc{};
+    for (int c = 0; c < h->nkeys; c++) {
+        kc.col[c] = keys[c];
+        kc.nm[c] = key_nulls ? key_nulls[c] : nullptr;
+    }
+    PredSet ps{};
+    QK_TRY(check_preds(npreds, preds, &ps));
+    // rows to aggregate again, rewritten in place each round (a row's entry only ever clears)
+    int8_t* again = nullptr;
+    for (;;) {
+        TableCounters c;
+        st = read_counters(h, s, &c);
+        if (st != QK_SUCCESS || c.err == 0) break;
+        QK_HIP(hipMemsetAsync(&h->ctr->err, 0, 2 * sizeof(int32_t), s));  // err and lost
+        if (c.err != QK_HASH_TABLE_FULL) {
+            st = fail(c.err, "GROUP BY kernel error %d", c.err);
+            break;
+        }
+        st = grow_table(h, c.nkeys + 1, s);
+        if (st != QK_SUCCESS) break;
+        if (!c.lost) continue;
+        if (!again) {
+            st = pool_alloc((void**)&again, (size_t)n);
+            if (st != QK_SUCCESS) break;
+        }
+        st = absent_rows(h, mode, kc, mask, ps, n, again, s);
+        if (st == QK_SUCCESS)
+            st = process_once(h, 1, keys, key_nulls, values, value_nulls, again, 0, nullptr, n, stream);
+        if (st != QK_SUCCESS) break;
+        mode = 1;
+        mask = again;
+    }
+    if (again) {
+        (void)hipStreamSynchronize(s);
+        pool_free(again);
+    }
+    return st;
 }
 
 int collect_profile(qk_groupby h) {
@@ -834,8 +1052,6 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     if (!key_types || !aggs) return fail(QK_ERROR, "qk_gb_create: null type/aggregate array");
     if (max_hash_count < 1 || key_size_multiplier < 1)
         return fail(QK_ERROR, "qk_gb_create: capacity must be positive");
-    if (flags & QK_GB_AUTO_GROW)
-        return fail(QK_ERROR, "qk_gb_create: QK_GB_AUTO_GROW is not available in this build");
     std::unique_ptr<qk_groupby_s> h(new qk_groupby_s());
     h->nkeys = nkeys;
     h->value_type = value_type;
@@ -918,42 +1134,9 @@ int qk_gb_create(qk_groupby* out, int32_t nkeys, const int32_t* key_types, int32
     }
     h->ks = ks;
 
-    const uint64_t gmax = (uint64_t)1 << 30;  // GB_BUFFER_SIZE_MAX (IGroupBy.h:6)
-    uint64_t logical = std::min<uint64_t>((uint64_t)max_hash_count * (uint64_t)key_size_multiplier, gmax);
-    h->logical_cap = logical;
-    h->reported_max = (int32_t)(nkeys == 1 ? logical + 1 : logical);  // + 1 for the NULL key
-    uint64_t cap = 64;
-    while (cap < 2 * logical) cap <<= 1;
-    h->cap = cap;
-    // sub-tables of at least 1024 slots, at most 256 of them (single-word keys only); more
-    // of them when that lets a sub-table fit the LDS, so that the two-phase path folds
-    // straight into its sub-tables.  Cold rows go to at most kMaxRecParts partitions, each
-    // holding the rows of up to 2^kMaxSplitBits sub-tables.
-    int32_t parts = 1;
-    if (ks.kw == 1) {
-        while (parts < 256 && cap / (uint64_t)(parts * 2) >= 1024) parts *= 2;
-        const size_t budget = std::min<size_t>(device_info().lds_per_block, 160 * 1024);
-        int32_t more = parts;
-        while (more < (kMaxRecParts << kMaxSplitBits) &&
-               p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) > budget &&
-               cap / (uint64_t)(more * 2) >= 1024)
-            more *= 2;
-        if (p2o_lds_bytes((int64_t)(cap / (uint64_t)more), fields) <= budget) parts = more;
-    }
-    h->parts = parts;
-    h->rec_parts = std::min<int32_t>(parts, kMaxRecParts);
-    const size_t slots = (size_t)(cap + 2);
-    int st = QK_SUCCESS;
-    auto A = [&](void** p, size_t bytes) {
-        if (st == QK_SUCCESS) st = pool_alloc(p, bytes);
-    };
-    A((void**)&h->keys, slots * ks.kw * 8);
-    A((void**)&h->cnt, slots * 8);
-    if (fields & F_CNTALL) A((void**)&h->cntall, slots * 8);
-    if (fields & F_SUM) A((void**)&h->sum, slots * 8);
-    if (fields & F_MIN) A((void**)&h->mn, slots * 8);
-    if (fields & F_MAX) A((void**)&h->mx, slots * 8);
-    A((void**)&h->ctr, sizeof(TableCounters));
+    layout_table(h.get(), std::min<uint64_t>((uint64_t)max_hash_count * (uint64_t)key_size_multiplier, kMaxLogical));
+    int st = alloc_slots(h.get());
+    if (st == QK_SUCCESS) st = pool_alloc((void**)&h->ctr, sizeof(TableCounters));
     if (st == QK_SUCCESS) {
         hipError_t e = hipGetDevice(&h->device);
         if (e != hipSuccess) st = fail(QK_RUNTIME_ERROR, "hipGetDevice: %s", hipGetErrorString(e));
@@ -1201,6 +1384,9 @@ int qk_gb_seed_keys(qk_groupby h, const void* keys, int32_t n, int32_t first_key
     if (h->nkeys != 1) return fail(QK_ERROR, "qk_gb_seed_keys: single-key tables only");
     if (n < 0 || (n > 0 && !keys)) return fail(QK_ERROR, "qk_gb_seed_keys: bad key buffer");
     if (h->seed_n > 0) return fail(QK_ERROR, "qk_gb_seed_keys: the table is already seeded");
+    if (h->flags & QK_GB_AUTO_GROW)
+        return fail(QK_ERROR, "qk_gb_seed_keys: a QK_GB_AUTO_GROW table moves its keys when it grows; "
+                    "seeded keys need a fixed-size table");
     if (n == 0) return QK_SUCCESS;
     hipStream_t s = as_stream(stream);
     QK_TRY(flush_pending(h, s));
@@ -1249,6 +1435,13 @@ int qk_gb_merge(qk_groupby dst, qk_groupby src, void* stream) {
     QK_TRY(flush_pending(src, s));
     QK_TRY(flush_pending(dst, s));
     QK_TRY(take_error(src, s));
+    if (dst->flags & QK_GB_AUTO_GROW) {
+        // room for every group of both tables before the merge starts
+        TableCounters cd, cs;
+        QK_TRY(read_counters(dst, s, &cd));
+        QK_TRY(read_counters(src, s, &cs));
+        if (cd.nkeys + cs.nkeys > dst->logical_cap) QK_TRY(grow_table(dst, cd.nkeys + cs.nkeys, s));
+    }
     GView gd = view_of(dst), gs = view_of(src);
     const int64_t grid = std::min<int64_t>((int64_t)((src->cap + 2 + 255) / 256), (int64_t)device_info().cus * 16);
     switch (dst->ks.kw) {
@@ -1306,7 +1499,8 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
     if (!h || !out || n < 0) return fail(QK_ERROR, "qk_gb_stats: bad argument");
     hipStream_t s = as_stream(stream);
     QK_HIP(hipStreamSynchronize(s));
-    int64_t v[11] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax, h->parts};
+    int64_t v[12] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax, h->parts,
+                     h->grows};
     if (h->tp.heavy_meta) {
         int32_t meta[2];
         QK_HIP(hipMemcpy(meta, h->tp.heavy_meta, 8, hipMemcpyDeviceToHost));
@@ -1318,7 +1512,7 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
         QK_HIP(hipMemcpy(rc.data(), h->tp.r_count, rc.size() * 4, hipMemcpyDeviceToHost));
         for (uint32_t c : rc) v[3] += c;
     }
-    for (int i = 0; i < n && i < 11; i++) out[i] = v[i];
+    for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
     return QK_SUCCESS;
 }
 

@@ -62,12 +62,13 @@ struct GView {
     unsigned long long* nkeys;  // distinct non-NULL keys inserted
     int32_t* err;
     uint64_t logical_cap;
+    int32_t* lost;              // set when a partial found no slot (its rows were dropped)
 };
 
 struct TableCounters {
     unsigned long long nkeys;
     int32_t err;
-    int32_t pad;
+    int32_t lost;
 };
 
 // Partial aggregate of one row, one LDS slot or one merge record.
@@ -200,6 +201,15 @@ __device__ __forceinline__ void g_set_error(const GView& g, int32_t code) {
     __hip_atomic_store(g.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A partial whose key found no free slot in its sub-table: the table reports itself full and
+// records that rows were dropped.  A sub-table never loses a key once it holds it, so the
+// dropped partials are exactly those whose key is absent from the table afterwards (every
+// partial of a key the table holds was aggregated); a growing table re-aggregates those rows.
+__device__ __forceinline__ void g_set_lost(const GView& g) {
+    __hip_atomic_store(g.lost, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    g_set_error(g, QK_HASH_TABLE_FULL);
+}
+
 // Side slot claim (0 -> 1); returns true when this call claimed it.
 __device__ __forceinline__ bool g_claim_side(const GView& g, uint64_t s) {
     uint64_t* kp = g.keys + s * (uint64_t)g.kw;
@@ -292,6 +302,37 @@ __device__ __forceinline__ int64_t g_slotN(const GView& g, const uint64_t (&k)[K
         s = next_in_sub(g, s);
     }
     return -1;
+}
+
+// Whether the table holds a key (no concurrent writers: a plain probe).  The NULL key and the
+// key equal to kEmpty of a single-word table live in their side slots.
+__device__ __forceinline__ bool g_has1(const GView& g, uint64_t key, bool knull, uint64_t h) {
+    if (knull) return g.keys[g.cap] != 0ull;
+    if (key == kEmpty) return g.keys[g.cap + 1] != 0ull;
+    uint64_t s = home_of(g, h);
+    for (uint64_t p = 0; p <= g.sub_mask; p++) {
+        const uint64_t cur = g.keys[s];
+        if (cur == key) return true;
+        if (cur == kEmpty) return false;
+        s = next_in_sub(g, s);
+    }
+    return false;
+}
+
+template <int KW>
+__device__ __forceinline__ bool g_hasN(const GView& g, const uint64_t (&k)[KW], uint64_t h) {
+    uint64_t s = home_of(g, h);
+    for (uint64_t p = 0; p <= g.sub_mask; p++) {
+        const uint64_t* kp = g.keys + s * KW;
+        const uint64_t last = kp[KW - 1];
+        if (last == 0ull) return false;
+        bool eq = last == k[KW - 1];
+#pragma unroll
+        for (int w = 0; w < KW - 1; w++) eq = eq && kp[w] == k[w];
+        if (eq) return true;
+        s = next_in_sub(g, s);
+    }
+    return false;
 }
 
 // Atomic merge of a partial into slot s (concurrent writers allowed).

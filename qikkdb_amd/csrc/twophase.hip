@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             bool is_new = false;
             const int64_t gs = g_slot1(g, kk, false, h, &is_new);
             if (is_new) atomicAdd(nnew, 1u);
-            if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+            if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_lost(g);
         }
     };
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -697,7 +697,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
         bool is_new = false;
         const int64_t gs = g_slot1(g, key, false, hash64(key), &is_new);
         if (is_new) atomicAdd(&misc[1], 1u);
-        if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_error(g, QK_HASH_TABLE_FULL);
+        if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_lost(g);
         atomicAdd(&misc[2], 1u);
     };
     auto add = [&](int s, uint32_t rows, uint32_t cnt, uint64_t sum, uint64_t mn, uint64_t mx) {
@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2(GView g, uint32_t fields, 
             bool is_new = false;
             const int64_t gs = g_slot1_claim(g, key, hash64(key), &is_new);
             if (is_new) atomicAdd(&misc[1], 1u);
-            if (gs < 0) g_set_error(g, QK_HASH_TABLE_FULL);
+            if (gs < 0) g_set_lost(g);
             else if (is_new) g_store_new(g, gs, part_of_slot(s), fields);
             else g_accum_owned(g, gs, part_of_slot(s), fields);
             clear_slot(s);
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
                 if (!rows) continue;
                 const int s = claim_from(key, (uint32_t)(kh & g.sub_mask));
                 if (s >= 0) add(s, (uint32_t)rows, (uint32_t)cnt, REAL ? (uint64_t)__double_as_longlong(dsum) : isum, mn, mx);
-                else g_set_error(g, QK_HASH_TABLE_FULL);
+                else g_set_lost(g);
             }
         }
     }
@@ -1099,7 +1099,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
                 int s = hit[i];
                 if (s < 0) s = claim_from(key[i], home[i]);
                 if (s >= 0) add(s, 1, 1, pr.sum, pr.mn, pr.mx);
-                else g_set_error(g, QK_HASH_TABLE_FULL);
+                else g_set_lost(g);
             }
         }
     }

@@ -260,7 +260,8 @@ class GPUGroupBy:
 
     def stats(self, stream=None) -> dict:
         names = ["path", "heavy_keys", "heavy_threshold", "region_rows", "p1_workgroups",
-                 "partitions", "region_cap", "p1_lds_slots", "p2_lds_slots", "heavy_limit", "sub_tables"]
+                 "partitions", "region_cap", "p1_lds_slots", "p2_lds_slots", "heavy_limit", "sub_tables",
+                 "grows"]
         v = (C.c_int64 * len(names))()
         call("qk_gb_stats", self.h, v, len(names), stream)
         return dict(zip(names, list(v)))
@@ -310,6 +311,10 @@ class GPUGroupBy:
 
     def check_error(self, stream=None) -> None:
         call("qk_gb_check_error", self.h, stream)
+
+    def merge(self, src: "GPUGroupBy", stream=None) -> None:
+        """Fold every group of `src` into this table (qk_gb_merge)."""
+        call("qk_gb_merge", self.h, src.h, stream)
 
     def group_count(self, stream=None) -> int:
         v = C.c_int64()
