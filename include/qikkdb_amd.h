@@ -136,6 +136,16 @@ int qk_logical_not(const int8_t* in, int32_t n, const uint64_t* null_mask, int8_
  * GPUReconstruct::PrefixSum / PrefixSumExclusive (GPUReconstruct.cuh:572-601). */
 int qk_prefix_sum(const int8_t* mask, int32_t n, int32_t* out, int32_t inclusive, void* stream);
 
+/* qk_prefix_sum without any allocation: its look-back state lives in a fixed device array of
+ * the library (about 2 MB per device, held until the process ends) instead of per-stream
+ * scratch, and the call makes no hipMalloc / hipFree.  No counterpart in the reference; it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops).
+ * At most 2^30 rows (QK_ERROR beyond).  qk_init sets the array up; without qk_init the first
+ * call on a device does, and synchronises the device then.  Calls on one device share the
+ * array: calls that overlap in time (other streams, other threads) race. */
+int qk_prefix_sum_noalloc(const int8_t* mask, int32_t n, int32_t* out, int32_t inclusive,
+                          void* stream);
+
 /* Sum of the int8 mask values, written to *out_host.  Replaces GPUReconstruct::Sum
  * (GPUReconstruct.cuh:604-624) on masks. */
 int qk_mask_sum(const int8_t* mask, int32_t n, int64_t* out_host, void* stream);

@@ -156,6 +156,7 @@ PROTOTYPES = {
     "qk_filter_bits": [_i32, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp, _vp],
     "qk_logical_not": [_vp, _i32, _vp, _vp, _vp],
     "qk_prefix_sum": [_vp, _i32, _vp, _i32, _vp],
+    "qk_prefix_sum_noalloc": [_vp, _i32, _vp, _i32, _vp],
     "qk_mask_sum": [_vp, _i32, _P(_i64), _vp],
     "qk_reconstruct_col_keep": [_i32, _vp, _vp, _i32, _vp, _P(_vp), _P(_i32), _P(_vp), _vp],
     "qk_generate_indexes_keep": [_vp, _i32, _i32, _P(_vp), _P(_i32), _vp],

@@ -57,6 +57,10 @@ int stream_scratch(hipStream_t s, size_t bytes, Scratch** out);
 // Small pinned host buffer per thread for count read-backs.
 int pinned_host(size_t bytes, void** out);
 
+// Resolves and clears the current device's fixed scratch of qk_prefix_sum_noalloc (once per
+// device; called by qk_init).
+int prefix_noalloc_setup();
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline int type_size(int32_t t) {

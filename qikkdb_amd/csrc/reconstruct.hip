@@ -7,6 +7,9 @@
 // ballots, so a column is compacted in one pass over mask + data without a materialised
 // prefix array.  Null bits are compacted per wave with one OR-reduced word pair instead of
 // one atomicOr per row.
+#include <map>
+#include <mutex>
+
 #include "lookback.hpp"
 #include "pred.hpp"
 
@@ -278,6 +281,64 @@ int launch_compact(const int8_t* mask, const PredSet& preds, const CompactCols& 
     return QK_SUCCESS;
 }
 
+// ---- prefix sum without allocation ----------------------------------------------------------
+// qk_prefix_sum_noalloc keeps its look-back state (ticket word + one status word per
+// 4096-row tile) in this fixed array instead of the per-stream scratch, so the call never
+// allocates.  Each device holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr int64_t kNoAllocMaxRows = (int64_t)1 << 30;
+constexpr int64_t kNoAllocTiles = kNoAllocMaxRows / kScanTile;
+constexpr size_t kNoAllocWords = kLbHeaderWords + kNoAllocTiles + 8;
+static __device__ __align__(256) uint64_t scratch[kNoAllocWords];
+
+struct NoAllocState {
+    uint64_t* words = nullptr;  // device address of `scratch` on this device
+    uint32_t epoch = 0;
+};
+std::mutex g_noalloc_mu;
+std::map<int, NoAllocState> g_noalloc;
+
+// Clears the ticket word before a launch (a kernel rather than a memset node, so the call can
+// be captured into a graph).
+__global__ void k_noalloc_arm(uint64_t* words) {
+    if (threadIdx.x < kLbHeaderWords) words[threadIdx.x] = 0;
+}
+
+__global__ void k_noalloc_clear(uint64_t* words, size_t nwords) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) words[i] = 0;
+}
+
+// Resolves this device's copy of `scratch` and clears it, once per device (the first call on
+// the device; it synchronises the device, so that call should not be made under capture).
+int noalloc_state(NoAllocState** out) {
+    int dev = 0;
+    QK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    NoAllocState& st = g_noalloc[dev];
+    if (!st.words) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(QK_RUNTIME_ERROR, "prefix_sum_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        k_noalloc_clear<<<256, 256>>>((uint64_t*)p, kNoAllocWords);
+        QK_HIP(hipGetLastError());
+        QK_HIP(hipDeviceSynchronize());
+        st.words = (uint64_t*)p;
+    }
+    *out = &st;
+    return QK_SUCCESS;
+}
+
+}  // namespace
+
+int prefix_noalloc_setup() {
+    NoAllocState* st = nullptr;
+    return noalloc_state(&st);
+}
+
+namespace {
+
 // Count non-zero mask bytes (kept rows) on the device and read the count back.
 int count_kept(const int8_t* mask, int64_t n, int64_t* out, hipStream_t s) {
     Scratch* sc = nullptr;
@@ -315,6 +376,31 @@ int qk_prefix_sum(const int8_t* mask, int32_t n, int32_t* out, int32_t inclusive
     const uint32_t epoch = ++sc->epoch;
     k_scan_mask<<<(unsigned)tiles, kScanBlock, 0, s>>>(mask, out, n, inclusive ? 1 : 0,
                                                        words + kLbHeaderWords, (uint32_t*)words,
+                                                       epoch, tiles);
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+int qk_prefix_sum_noalloc(const int8_t* mask, int32_t n, int32_t* out, int32_t inclusive, void* stream) {
+    QK_TRY(ensure_init());
+    if (n <= 0) return QK_SUCCESS;
+    if (!mask || !out) return fail(QK_ERROR, "prefix_sum_noalloc: null buffer");
+    if ((int64_t)n > kNoAllocMaxRows)
+        return fail(QK_ERROR, "prefix_sum_noalloc: %d rows exceed the fixed scratch's maximum of %lld rows",
+                    n, (long long)kNoAllocMaxRows);
+    NoAllocState* st = nullptr;
+    QK_TRY(noalloc_state(&st));
+    uint32_t epoch;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        epoch = ++st->epoch;
+    }
+    hipStream_t s = as_stream(stream);
+    const int64_t tiles = ((int64_t)n + kScanTile - 1) / kScanTile;
+    k_noalloc_arm<<<1, 64, 0, s>>>(st->words);
+    QK_HIP(hipGetLastError());
+    k_scan_mask<<<(unsigned)tiles, kScanBlock, 0, s>>>(mask, out, n, inclusive ? 1 : 0,
+                                                       st->words + kLbHeaderWords, (uint32_t*)st->words,
                                                        epoch, tiles);
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;

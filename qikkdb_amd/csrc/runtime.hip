@@ -204,7 +204,8 @@ int qk_init(int device) {
     if (device < 0 || device >= n)
         return fail(QK_ERROR, "qk_init: device %d out of range (%d devices)", device, n);
     QK_HIP(hipSetDevice(device));
-    return ensure_init();
+    QK_TRY(ensure_init());
+    return prefix_noalloc_setup();
 }
 
 int qk_device_count(int* count) {
