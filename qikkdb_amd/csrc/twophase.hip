@@ -267,11 +267,76 @@ __global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ c
     }
 }
 
+// ---- heavy table image -------------------------------------------------------------------
+
+// The bucket of a key in the heavy table: a multiplicative hash of the key's two halves folded
+// together (Knuth's golden-ratio multiplier; the high bits of the product pick the bucket).
+// The table only needs keys spread over the buckets, so this stands in for the full 64-bit
+// hash on the path every row takes.
+__device__ __forceinline__ uint32_t bucket_of(uint64_t k, uint32_t nb) {
+    const uint32_t f = ((uint32_t)k ^ (uint32_t)(k >> 32)) * 0x9e3779b1u;
+    return __umulhi(f, nb);
+}
+
+// One workgroup, once per heavy set: places each heavy key in a free slot of its bucket of an
+// H-slot table, the most frequent keys first (one round per power of two of the sample count),
+// so that the keys carrying most rows always find a slot.  A key that finds its bucket full
+// stays cold (its rows then go through its partition like any other key's).  The table's keys
+// go to image[H] (kEmpty in free slots) and each heavy key's slot to slot[k] (-1 when not
+// placed); every phase-1 workgroup starts from this image instead of placing the keys itself.
+__global__ void __launch_bounds__(1024) k_tp_place(const uint64_t* __restrict__ heavy,
+                                                   const uint32_t* __restrict__ heavy_cnt,
+                                                   const int32_t* __restrict__ meta, int32_t kmax, int32_t H,
+                                                   uint64_t* __restrict__ image, int32_t* __restrict__ slot) {
+    extern __shared__ __align__(16) uint64_t lds[];
+    uint64_t* lkey = lds;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const uint32_t nb = (uint32_t)(H / kBucket);
+    for (int i = tid; i < H; i += bd) lkey[i] = kEmpty;
+    __syncthreads();
+    const int32_t K = min(meta[0], kmax);
+    // kmax <= 4 * blockDim, so each thread holds at most kPlace heavy keys
+    constexpr int kPlace = 4;
+    static_assert(kPlace * 1024 >= 4096, "heavy keys per thread");
+    uint64_t pkey[kPlace];
+    int pcls[kPlace];
+#pragma unroll
+    for (int i = 0; i < kPlace; i++) {
+        const int k = tid + i * bd;
+        pcls[i] = -1;
+        pkey[i] = 0;
+        if (k < K) {
+            pkey[i] = heavy[k];
+            pcls[i] = 31 - __builtin_clz(max(heavy_cnt[k], 1u));
+        }
+    }
+    for (int cls = 31; cls >= 0; cls--) {
+        bool mine = false;
+#pragma unroll
+        for (int i = 0; i < kPlace; i++) mine = mine || pcls[i] == cls;
+        if (!__syncthreads_or(mine)) continue;
+#pragma unroll
+        for (int i = 0; i < kPlace; i++) {
+            if (pcls[i] != cls) continue;
+            const uint32_t bb = bucket_of(pkey[i], nb);
+            int placed = -1;
+            for (int j = 0; j < kBucket && placed < 0; j++) {
+                const unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
+                                                         (unsigned long long)kEmpty, (unsigned long long)pkey[i]);
+                if (old == kEmpty) placed = (int)(bb * kBucket + j);
+            }
+            slot[tid + i * bd] = placed;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < H; i += bd) image[i] = lkey[i];
+}
+
 // ---- phase 1 -------------------------------------------------------------------------------
 
 struct P1Args {
-    const uint64_t* heavy;
-    const uint32_t* heavy_cnt;  // sample count of each heavy key
+    const uint64_t* image;      // [H] keys of the heavy table (k_tp_place)
+    const int32_t* slot;        // [kmax] slot of each heavy key in it, -1 if not placed
     const int32_t* meta;
     int32_t kmax, H, P, cap;
     uint32_t* p_cnt;
@@ -286,24 +351,21 @@ struct P1Args {
     int32_t accumulate;    // regions and partials hold rows of earlier blocks
 };
 
-// H hashed slots in H/kBucket buckets, + 2 side slots for the partial arrays.
-__host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, int32_t kmax, uint32_t fields) {
+// Cold rows wait in a ring of kQueue entries per wavefront until 64 of them are there, then
+// the whole wavefront partitions and writes those 64 at once.
+constexpr int kQueue = 128;
+
+// LDS of phase 1: H hashed slots in H/kBucket buckets + 2 side slots for the partial arrays,
+// the cold-row rings of the workgroup's wavefronts (key + value of vbytes), the partition
+// cursors.
+__host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, int32_t vbytes, uint32_t fields) {
     const size_t T = (size_t)H + 2;
     size_t b = (size_t)H * 8;                                  // keys
     b += T * 8 * (((fields & F_SUM) ? 1 : 0) + ((fields & F_MIN) ? 1 : 0) + ((fields & F_MAX) ? 1 : 0));
     b += T * 4 * 2;                                            // cnt, rows
-    b += (size_t)kmax * 4;                                     // slot of each heavy key
+    b += (size_t)(kBlock / 64) * kQueue * (8 + (size_t)vbytes); // cold-row rings
     b += (size_t)P * 4 + 16;                                   // cursors, counters
     return b;
-}
-
-// The bucket of a key in the heavy table: a multiplicative hash of the key's two halves folded
-// together (Knuth's golden-ratio multiplier; the high bits of the product pick the bucket).
-// The table only needs keys spread over the buckets, so this stands in for the full 64-bit
-// hash on the path every row takes.
-__device__ __forceinline__ uint32_t bucket_of(uint64_t k, uint32_t nb) {
-    const uint32_t f = ((uint32_t)k ^ (uint32_t)(k >> 32)) * 0x9e3779b1u;
-    return __umulhi(f, nb);
 }
 
 // Slot of key k in the heavy table, or -1 (one bucket of kBucket keys, two 128-bit reads).
@@ -348,10 +410,12 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     extern __shared__ __align__(16) uint64_t lds[];
     constexpr bool HAS_V = VTraits<VT>::has;
     constexpr bool REAL = VTraits<VT>::real;
+    using VV = typename std::conditional<HAS_V, VT, int>::type;
     const bool f_sum = FP == kFieldsSum ? true : (fields & F_SUM) != 0;
     const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
     const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
     const int H = a.H, T = H + 2, bd = blockDim.x, tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const uint32_t nb = (uint32_t)(H / kBucket);
     uint64_t* lkey = lds;
     uint64_t* p = lkey + H;
@@ -361,14 +425,23 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     p += f_min ? T : 0;
     uint64_t* lmax = f_max ? p : nullptr;
     p += f_max ? T : 0;
+    // the cold-row rings of the wavefronts: keys [waves][kQueue], then values
+    uint64_t* qkey = p + (size_t)wave * kQueue;
+    p += (size_t)(kBlock / 64) * kQueue;
+    VV* qval = reinterpret_cast<VV*>(p) + (size_t)wave * kQueue;
+    p = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(p) + (size_t)(kBlock / 64) * kQueue * (HAS_V ? sizeof(VT) : 0));
     uint32_t* lcnt = (uint32_t*)p;   // rows with a non-NULL value
     uint32_t* lnul = lcnt + T;       // rows with a NULL value
-    int32_t* slot_of = (int32_t*)(lnul + T);  // [kmax]
-    uint32_t* cursor = (uint32_t*)(slot_of + a.kmax);
+    uint32_t* cursor = lnul + T;
     uint32_t* nnew = cursor + a.P;
 
+    // the heavy table's keys from the image (two keys per 16-byte load), states cleared
+    {
+        const ulonglong2* img = reinterpret_cast<const ulonglong2*>(a.image);
+        ulonglong2* dst = reinterpret_cast<ulonglong2*>(lkey);
+        for (int i = tid; i < H / 2; i += bd) dst[i] = img[i];
+    }
     for (int s = tid; s < T; s += bd) {
-        if (s < H) lkey[s] = kEmpty;
         lcnt[s] = 0;
         lnul[s] = 0;
         if (lsum) lsum[s] = 0;
@@ -380,51 +453,12 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     for (int q = tid; q < a.P; q += bd) cursor[q] = a.accumulate ? a.r_count[region0 + q] : 0u;
     if (tid == 0) nnew[0] = 0;
     __syncthreads();
-    // Place each heavy key in a free slot of its bucket, the most frequent keys first (one
-    // round per power of two of the sample count), so that the keys carrying most rows always
-    // find a slot.  A key that finds its bucket full stays cold: its rows then go through its
-    // partition like any other key's.
     const int32_t K = min(a.meta[0], a.kmax);
-    // kmax <= 4 * blockDim, so each thread holds at most kPlace heavy keys
-    constexpr int kPlace = 4;
-    static_assert(kPlace * kBlock >= 4096, "heavy keys per thread");
-    uint64_t pkey[kPlace];
-    int pcls[kPlace];
-#pragma unroll
-    for (int i = 0; i < kPlace; i++) {
-        const int k = tid + i * bd;
-        pcls[i] = -1;
-        pkey[i] = 0;
-        if (k < K) {
-            pkey[i] = a.heavy[k];
-            pcls[i] = 31 - __builtin_clz(max(a.heavy_cnt[k], 1u));
-        }
-    }
-    for (int cls = 31; cls >= 0; cls--) {
-        bool mine = false;
-#pragma unroll
-        for (int i = 0; i < kPlace; i++) mine = mine || pcls[i] == cls;
-        if (!__syncthreads_or(mine)) continue;
-#pragma unroll
-        for (int i = 0; i < kPlace; i++) {
-            if (pcls[i] != cls) continue;
-            const uint32_t bb = bucket_of(pkey[i], nb);
-            int placed = -1;
-            for (int j = 0; j < kBucket && placed < 0; j++) {
-                const unsigned long long old = atomicCAS((unsigned long long*)&lkey[bb * kBucket + j],
-                                                         (unsigned long long)kEmpty, (unsigned long long)pkey[i]);
-                if (old == kEmpty) placed = (int)(bb * kBucket + j);
-            }
-            slot_of[tid + i * bd] = placed;
-        }
-    }
-    __syncthreads();
 
     const int64_t tile_rows = (int64_t)bd * kP1Steps;
     const int64_t ntiles = (n + tile_rows - 1) / tile_rows;
     // rows of one wave in one step are 64 consecutive rows starting at a multiple of 64, so
     // they share one word of each null bitmap, loaded once for the wave
-    const int lane = tid & 63;
     const int wave_row = __builtin_amdgcn_readfirstlane(tid & ~63);
     // loads are unconditional so that a tile's loads are all in flight together: rows past
     // the end re-read the last row (and are dropped below), and a missing bitmap reads a
@@ -433,7 +467,6 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
     const int64_t vstep = vnull ? 1 : 0;
     const uint64_t* kbits = kc.nm[0] ? kc.nm[0] : a.zero;
     const int64_t kstep = kc.nm[0] ? 1 : 0;
-    using VV = typename std::conditional<HAS_V, VT, int>::type;
     const uint64_t* cbits = preds.rnull ? preds.rnull : a.zero;
     const int64_t cstep = preds.rnull ? 1 : 0;
     struct Tile {
@@ -502,52 +535,92 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             o.knull |= (uint32_t)kn << k;
         }
     };
+    // A row that cannot go to a region (NULL value, or its region is full): straight into the
+    // global table with atomics.
+    auto atomic_row = [&](uint64_t kk, uint64_t h, const VV& v, bool vn) {
+        Part pr;
+        if constexpr (HAS_V) pr = row_part<VT>(v, vn); else pr = row_part<NoValue>(NoValue{}, vn);
+        const GView& g = *gp;
+        bool is_new = false;
+        const int64_t gs = g_slot1(g, kk, false, h, &is_new);
+        if (is_new) atomicAdd(nnew, 1u);
+        if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_lost(g);
+    };
+    // This wavefront's ring: head and count are the same on every lane (they only change by
+    // ballot counts), so the ring's control flow is uniform.
+    uint32_t qhead = 0, qcnt = 0;
+    const uint64_t below = (1ull << lane) - 1ull;
+    // Partition and write the ring's first `cnt` (<= 64) rows, one per lane.
+    auto drain = [&](uint32_t cnt) {
+        const bool live = (uint32_t)lane < cnt;
+        const uint32_t at_q = (qhead + (uint32_t)lane) & (kQueue - 1);
+        const uint64_t kk = qkey[at_q];
+        VV v = 0;
+        if constexpr (HAS_V) v = qval[at_q];
+        const uint64_t h = hash64(kk);
+        // part_shift is 54..64; the split shift gives 0 for 64 without a select
+        const uint32_t q = (uint32_t)((h >> 1) >> (a.part_shift - 1));
+        uint32_t c = ~0u;
+        if (live) c = atomicAdd(&cursor[q], 1u);
+        if (live && c < (uint32_t)a.cap) {
+            put_rec<VT>(a.r_rec, (region0 + q) * (int64_t)a.cap + c, kk, v);
+        } else if (live) {
+            atomic_row(kk, h, v, false);
+        }
+        qhead = (qhead + cnt) & (kQueue - 1);
+    };
     auto aggregate_rows = [&](const Rows& o) {
+        // every step's bucket is read first (a NULL key reads key 0's; the NULL key and the
+        // key equal to the empty marker then take their side slots by selection)
+        int ls[kP1Steps];
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
-            if (!((o.pass >> k) & 1)) continue;
-            const bool vn = (o.vnull >> k) & 1, kn = (o.knull >> k) & 1;
             const uint64_t kk = o.key[k];
-            // the bucket is read for every row (a NULL key reads key 0's); the NULL key and the
-            // key equal to the empty marker then take their side slots by selection
-            int ls = heavy_find(lkey, nb, kk);
-            ls = kk == kEmpty ? H + 1 : ls;
-            ls = kn ? H : ls;
-            Part pr;
-            if constexpr (HAS_V) pr = row_part<VT>(o.val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
-            if (ls >= 0) {
+            int l = heavy_find(lkey, nb, kk);
+            l = kk == kEmpty ? H + 1 : l;
+            ls[k] = ((o.knull >> k) & 1) ? H : l;
+        }
+#pragma unroll
+        for (int k = 0; k < kP1Steps; k++) {
+            const bool pass = (o.pass >> k) & 1, vn = (o.vnull >> k) & 1;
+            if (pass && ls[k] >= 0) {
+                Part pr;
+                if constexpr (HAS_V) pr = row_part<VT>(o.val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
+                const int l = ls[k];
                 if (pr.cnt) {
-                    atomicAdd(&lcnt[ls], 1u);
+                    atomicAdd(&lcnt[l], 1u);
                     if (lsum) {
                         if (REAL)
-                            atomicAdd((double*)&lsum[ls], __longlong_as_double((long long)pr.sum));
+                            atomicAdd((double*)&lsum[l], __longlong_as_double((long long)pr.sum));
                         else
-                            atomicAdd((unsigned long long*)&lsum[ls], (unsigned long long)pr.sum);
+                            atomicAdd((unsigned long long*)&lsum[l], (unsigned long long)pr.sum);
                     }
-                    if (lmin) atomicMin((unsigned long long*)&lmin[ls], (unsigned long long)pr.mn);
-                    if (lmax) atomicMax((unsigned long long*)&lmax[ls], (unsigned long long)pr.mx);
+                    if (lmin) atomicMin((unsigned long long*)&lmin[l], (unsigned long long)pr.mn);
+                    if (lmax) atomicMax((unsigned long long*)&lmax[l], (unsigned long long)pr.mx);
                 } else {
-                    atomicAdd(&lnul[ls], 1u);
-                }
-                continue;
-            }
-            const uint64_t h = hash64(kk);
-            if (!vn) {
-                // part_shift is 56..64; the split shift gives 0 for 64 without a select
-                const uint32_t q = (uint32_t)((h >> 1) >> (a.part_shift - 1));
-                const uint32_t c = atomicAdd(&cursor[q], 1u);
-                if (c < (uint32_t)a.cap) {
-                    const int64_t at = (region0 + q) * (int64_t)a.cap + c;
-                    put_rec<VT>(a.r_rec, at, kk, o.val[k]);
-                    continue;
+                    atomicAdd(&lnul[l], 1u);
                 }
             }
-            // NULL value or full region: atomic path
-            const GView& g = *gp;
-            bool is_new = false;
-            const int64_t gs = g_slot1(g, kk, false, h, &is_new);
-            if (is_new) atomicAdd(nnew, 1u);
-            if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_lost(g);
+        }
+        // cold rows with a value go to the ring; a NULL value takes the atomic path
+#pragma unroll
+        for (int k = 0; k < kP1Steps; k++) {
+            const bool pass = (o.pass >> k) & 1, vn = (o.vnull >> k) & 1;
+            const bool cold = pass & (ls[k] < 0);
+            if (cold && vn) atomic_row(o.key[k], hash64(o.key[k]), o.val[k], true);
+            const uint64_t m = __ballot(cold && !vn);
+            if (m) {
+                if (cold && !vn) {
+                    const uint32_t at_q = (qhead + qcnt + (uint32_t)__popcll(m & below)) & (kQueue - 1);
+                    qkey[at_q] = o.key[k];
+                    if constexpr (HAS_V) qval[at_q] = o.val[k];
+                }
+                qcnt += (uint32_t)__popcll(m);
+                if (qcnt >= 64) {
+                    drain(64);
+                    qcnt -= 64;
+                }
+            }
         }
     };
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -557,13 +630,14 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
         filter_tile(tile, t, o);
         aggregate_rows(o);
     }
+    if (qcnt) drain(qcnt);
 
     __syncthreads();
     const int64_t pbase = (int64_t)blockIdx.x * (a.kmax + 2);
-    // heavy partials, indexed by heavy key (zeros for a key this workgroup could not place),
+    // heavy partials, indexed by heavy key (zeros for a key the image could not place),
     // added to what earlier blocks left when phase 2 has not consumed it yet
     for (int k = tid; k < K + 2; k += bd) {
-        const int s = k < K ? slot_of[k] : H + (k - K);
+        const int s = k < K ? a.slot[k] : H + (k - K);
         const int64_t at = pbase + (k < K ? k : a.kmax + (k - K));
         uint32_t cnt = s >= 0 ? lcnt[s] : 0u, rows = s >= 0 ? lcnt[s] + lnul[s] : 0u;
         uint64_t sum = (lsum && s >= 0) ? lsum[s] : 0ull;
@@ -1234,8 +1308,8 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
               int64_t n, uint32_t fields, hipStream_t s) {
     // gfx950 launches up to 160 KB of LDS per workgroup without a function attribute
     P1Args a;
-    a.heavy = b.heavy_keys;
-    a.heavy_cnt = b.heavy_cnt;
+    a.image = b.p1_image;
+    a.slot = b.p1_slot;
     a.meta = b.heavy_meta;
     a.kmax = b.kmax;
     a.H = b.H1;
@@ -1251,7 +1325,7 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
     a.zero = b.zero_word;
     a.part_shift = rec_shift_of(b.P);
     a.accumulate = b.pending_rows > 0 ? 1 : 0;
-    const size_t lds = p1_lds_bytes(b.H1, b.P, b.kmax, fields);
+    const size_t lds = p1_lds_bytes(b.H1, b.P, b.vbytes, fields);
     k_tp_phase1<VT, MODE, KT, FP><<<b.W, b.block, lds, s>>>(b.gview, ks, kc, (const VT*)vals, vnull,
                                                             mask, ps, vpred, n, fields, a);
     QK_HIP(hipGetLastError());
@@ -1346,6 +1420,7 @@ bool twophase_supported(int32_t kw, int32_t value_type) {
 
 void twophase_free(TwoPhaseBufs& b) {
     void* ptrs[] = {b.heavy_keys, b.heavy_cnt, b.cand_cnt, b.heavy_meta, b.sample_keys, b.sample_cnt, b.hist, b.cand,
+                    b.p1_image, b.p1_slot,
                     b.p_cnt, b.p_rows, b.p_sum, b.p_min, b.p_max, b.r_rec, b.r_count,
                     b.zero_word, b.gview};
     for (void* p : ptrs)
@@ -1393,7 +1468,7 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     // phase-1 LDS: buckets of kBucket slots, at most half of the slots hold heavy keys
     int32_t H1 = 0;
     for (int32_t h = kBucket; ; h += kBucket) {
-        if (p1_lds_bytes(h, parts, std::min(h / 2, 4096), fields) > lds) break;
+        if (p1_lds_bytes(h, parts, vbytes, fields) > lds) break;
         H1 = h;
     }
     b.H1 = H1;
@@ -1424,6 +1499,8 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     A((void**)&b.sample_cnt, (size_t)sslots * 4);
     A((void**)&b.hist, (size_t)kHistBins * 4);
     A((void**)&b.cand, (size_t)b.kmax * 8);
+    A((void**)&b.p1_image, (size_t)b.H1 * 8);
+    A((void**)&b.p1_slot, (size_t)b.kmax * 4);
     A((void**)&b.p_cnt, (size_t)b.W * kp * 4);
     A((void**)&b.p_rows, (size_t)b.W * kp * 4);
     if (fields & F_SUM) A((void**)&b.p_sum, (size_t)b.W * kp * 8);
@@ -1440,6 +1517,10 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     // no heavy set until the next sample: K = 0, every partition empty
     QK_HIP(hipMemsetAsync(b.heavy_meta, 0, (size_t)(parts + 8) * 4, s));
     QK_HIP(hipMemsetAsync(b.zero_word, 0, 64, s));
+    // the image of the empty heavy set
+    k_tp_place<<<1, 1024, (size_t)b.H1 * 8, s>>>(b.heavy_keys, b.heavy_cnt, b.heavy_meta, b.kmax, b.H1,
+                                                 b.p1_image, b.p1_slot);
+    QK_HIP(hipGetLastError());
     b.heavy_ready = false;
     return QK_SUCCESS;
 }
@@ -1472,6 +1553,8 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
                                                                b.kmax, b.heavy_meta, b.cand, b.cand_cnt);
     k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.cand_cnt, b.kmax, b.P, rec_shift_of(b.P), b.heavy_meta, b.heavy_keys,
                                  b.heavy_cnt);
+    k_tp_place<<<1, 1024, (size_t)b.H1 * 8, s>>>(b.heavy_keys, b.heavy_cnt, b.heavy_meta, b.kmax, b.H1,
+                                                 b.p1_image, b.p1_slot);
     QK_HIP(hipGetLastError());
     b.heavy_ready = true;
     return QK_SUCCESS;

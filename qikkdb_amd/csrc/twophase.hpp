@@ -42,6 +42,8 @@ struct TwoPhaseBufs {
     uint32_t* hist = nullptr;         // [kHistBins]
     uint64_t* cand = nullptr;         // [kmax] unsorted heavy candidates
     uint32_t* cand_cnt = nullptr;     // [kmax] their sample counts
+    uint64_t* p1_image = nullptr;     // [H1] keys of the phase-1 heavy table, built once per heavy set
+    int32_t* p1_slot = nullptr;       // [kmax] slot of each heavy key in it (-1: not placed)
     // per phase-1 workgroup partials [W][kmax + 2]
     uint32_t* p_cnt = nullptr;
     uint32_t* p_rows = nullptr;
