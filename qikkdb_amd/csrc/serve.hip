@@ -480,6 +480,68 @@ int qk_serve_replay(qk_serve s, int32_t n, const qk_batch_query* q, const double
     return QK_SUCCESS;
 }
 
+int qk_serve_replay_plain(const qk_batch_columns* cols, int32_t n, const qk_batch_query* q,
+                          const double* arrival_ms, double* latency_ms, void* stream) {
+    if (!cols || n < 0 || (n > 0 && (!q || !arrival_ms || !latency_ms)))
+        return qk::fail(QK_ERROR, "qk_serve_replay_plain: bad argument");
+    if (!cols->a || !cols->b || !cols->ts || !cols->value || cols->rows <= 0 || cols->block_rows <= 0)
+        return qk::fail(QK_ERROR, "qk_serve_replay_plain: bad columns");
+    for (int32_t i = 1; i < n; i++)
+        if (arrival_ms[i] < arrival_ms[i - 1])
+            return qk::fail(QK_ERROR, "qk_serve_replay_plain: arrivals must not decrease");
+    const int64_t nblocks = (cols->rows + cols->block_rows - 1) / cols->block_rows;
+    for (int32_t i = 0; i < n; i++) {
+        latency_ms[i] = -1.0;
+        if (q[i].key_mod < 1 || q[i].nblocks < 1 || q[i].first_block < 0 || q[i].first_block + (int64_t)q[i].nblocks > nblocks)
+            return qk::fail(QK_ERROR, "qk_serve_replay_plain: request %d out of range", i);
+    }
+    QK_TRY(qk::ensure_init());
+    void* keybuf = nullptr;
+    QK_TRY(qk::pool_alloc(&keybuf, (size_t)cols->block_rows * 8));
+    const int32_t key_type = QK_LONG;
+    const int32_t aggs[4] = {QK_AGG_SUM, QK_AGG_MIN, QK_AGG_MAX, QK_AGG_COUNT};
+    const auto t0 = Clock::now();
+    auto ms_since = [&] { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); };
+    int st = QK_SUCCESS;
+    for (int32_t i = 0; i < n && st == QK_SUCCESS; i++) {
+        const double wait = arrival_ms[i] - ms_since();
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
+        const qk_batch_query& r = q[i];
+        qk_groupby h = nullptr;
+        st = qk_gb_create(&h, 1, &key_type, QK_LONG, 4, aggs, nullptr, r.key_mod, 1, QK_GB_KEYS_NOT_NULL);
+        for (int64_t bi = r.first_block; st == QK_SUCCESS && bi < r.first_block + r.nblocks; bi++) {
+            const int64_t off = bi * cols->block_rows;
+            const int32_t rows = (int32_t)std::min<int64_t>(cols->block_rows, cols->rows - off);
+            st = qk_batch_key_column(cols->a + off, cols->b + off, cols->b_range, r.key_mod, rows,
+                                     (int64_t*)keybuf, stream);
+            if (st != QK_SUCCESS) break;
+            qk_predicate p[2];
+            std::memset(p, 0, sizeof(p));
+            for (int k = 0; k < 2; k++) {
+                p[k].op = k == 0 ? QK_GREATER_EQUAL : QK_LESS;
+                p[k].type = QK_LONG;
+                p[k].column = cols->ts + off;
+                p[k].rhs_kind = QK_CONST;
+                p[k].rhs_type = QK_LONG;
+                p[k].rhs_int = k == 0 ? r.ts_lo : r.ts_hi;
+            }
+            const void* kp = keybuf;
+            st = qk_gb_process_block_where(h, &kp, nullptr, cols->value + off, nullptr, rows, 2, p, stream);
+        }
+        void* okeys[1] = {nullptr};
+        void* ovals[4] = {nullptr, nullptr, nullptr, nullptr};
+        int32_t groups = 0;
+        if (st == QK_SUCCESS) st = qk_gb_get_results(h, okeys, nullptr, ovals, nullptr, &groups, stream);
+        if (st == QK_SUCCESS) st = qk_stream_synchronize(stream);
+        if (st == QK_SUCCESS) latency_ms[i] = ms_since() - arrival_ms[i];
+        qk_free(okeys[0]);
+        for (void* v : ovals) qk_free(v);
+        if (h) qk_gb_destroy(h);
+    }
+    qk::pool_free(keybuf);
+    return st;
+}
+
 int qk_serve_get_stats(qk_serve s, qk_serve_stats* out) {
     if (!s || !out) return qk::fail(QK_ERROR, "qk_serve_get_stats: null argument");
     std::lock_guard<std::mutex> lk(s->mu);

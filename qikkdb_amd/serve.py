@@ -115,4 +115,23 @@ class Server:
         return {k: getattr(s, k) for k, _ in ServeStats._fields_}
 
 
-__all__ = ["Config", "Server"]
+def replay_plain(a, b, ts, value, rows: int, block_rows: int, b_range: int, qs: Sequence[Query],
+                 arrival_ms: np.ndarray, stream=None) -> np.ndarray:
+    """The plain one-request-at-a-time path under the same open-loop arrivals
+    (qk_serve_replay_plain): each request's latency, arrival to groups in device memory, in ms."""
+    from ._abi import BatchQuery
+
+    ensure_device()
+    cols = BatchColumns()
+    cols.a, cols.b, cols.ts, cols.value = int(a), int(b), int(ts), int(value)
+    cols.rows, cols.block_rows, cols.b_range = rows, block_rows, b_range
+    n = len(qs)
+    arr = (BatchQuery * max(1, n))(*[q.as_c() for q in qs])
+    arrival = np.ascontiguousarray(arrival_ms, np.float64)
+    lat = np.zeros(n, np.float64)
+    call("qk_serve_replay_plain", C.byref(cols), n, arr, arrival.ctypes.data_as(C.POINTER(C.c_double)),
+         lat.ctypes.data_as(C.POINTER(C.c_double)), stream)
+    return lat
+
+
+__all__ = ["Config", "Server", "replay_plain"]

@@ -163,3 +163,36 @@ def test_open_loop_replay(table):
         st = srv.stats()
     assert (lat >= 0).all() and st["served"] == len(qs) and st["failed"] == 0
     assert st["batches"] >= 2
+
+
+def test_served_rate_follows_offered_below_capacity(table):
+    # capacity from a closed-loop replay (every request queued at once), then Poisson arrivals
+    # at 40% of it: the loop keeps up, so the served rate is the offered rate
+    _, cols = table
+    with S.Server(*ptrs(cols), N, BLOCK, B_RANGE) as srv:
+        qs = make_queries(400, seed=12)
+        srv.replay(qs, np.zeros(len(qs)))  # warm-up
+        lat = srv.replay(qs, np.zeros(len(qs)))
+        capacity = len(qs) / (lat.max() / 1e3)
+        offered = min(0.4 * capacity, 3000.0)
+        m = int(offered * 1.5)
+        rng = np.random.default_rng(13)
+        arrival = np.cumsum(rng.exponential(1000.0 / offered, m))
+        qs = make_queries(m, seed=14)
+        print(f"case: capacity {capacity:.0f} q/s, offered {offered:.0f} q/s, {m} requests", flush=True)
+        lat = srv.replay(qs, arrival)
+    served = m / ((arrival + lat).max() / 1e3)
+    assert (lat >= 0).all()
+    assert served >= 0.9 * offered, (served, offered, capacity)
+
+
+def test_plain_replay_serves_in_arrival_order(table):
+    _, cols = table
+    qs = make_queries(24, seed=15)
+    arrival = np.linspace(0.0, 20.0, len(qs))
+    lat = S.replay_plain(*ptrs(cols), N, BLOCK, B_RANGE, qs, arrival)
+    done = arrival + lat
+    assert (lat >= 0).all()
+    assert (np.diff(done) >= 0).all()  # one worker, first come first served
+    with pytest.raises(qk.QueryEngineError):
+        S.replay_plain(*ptrs(cols), N, BLOCK, B_RANGE, qs[:2], np.array([5.0, 1.0]))
