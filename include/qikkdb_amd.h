@@ -91,6 +91,18 @@ int qk_mem_stats(size_t* in_use, size_t* cached);
 /* Return every cached block to the driver. */
 int qk_mem_release_cached(void);
 
+/* Cap on the device memory the pool holds from the driver (blocks in use plus cached ones;
+ * 0 = no cap).  Past it qk_malloc first returns cached blocks to the driver, then fails with
+ * QK_OUT_OF_MEMORY.  Replaces the GPUMemoryUsagePercent reservation of CudaMemAllocator
+ * (CudaMemAllocator.cu:109-152); qk_mem_set_limit_percent takes the same percentage, of the
+ * memory free now plus what the pool already holds.  Per device (the calling thread's). */
+int qk_mem_set_limit(size_t bytes);
+int qk_mem_set_limit_percent(int32_t percent);
+int qk_mem_get_limit(size_t* bytes);
+/* Whether qk_malloc(bytes) would succeed now: CudaMemAllocator::CanAllocate
+ * (CudaMemAllocator.h:57). */
+int qk_mem_can_allocate(size_t bytes, int32_t* out);
+
 /* Copies: GPUMemory::copyHostToDevice / copyDeviceToHost / copyDeviceToDevice
  * (GPUMemory.cuh:113-189).  qk_memcpy_dtoh returns after the bytes are on the host. */
 int qk_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
@@ -102,6 +114,34 @@ int qk_stream_create(void** stream);
 int qk_stream_destroy(void* stream);
 int qk_stream_synchronize(void* stream);
 int qk_device_synchronize(void);
+
+/* ---- block cache: GPUMemoryCache (GPUMemoryCache.h:10-141, GPUMemoryCache.cpp:13-117) ----
+ * Device blocks named by (database, table.column, block index, load size, load offset), kept
+ * in least-recently-used order under max_bytes and the pool's cap.  A miss allocates the block
+ * (from qk_malloc; its contents are the caller's to fill), evicting least recently used blocks
+ * first; blocks of the current block index whose "database.table.column" is on the lock list
+ * are never evicted.  The cache owns its blocks: they stay valid until evicted, cleared or the
+ * cache is destroyed, and the device is synchronised before a block is given up. */
+typedef struct qk_cache_s* qk_cache;
+/* On the calling thread's device (GPUMemoryCache(deviceID, maximumSize)). */
+int qk_cache_create(qk_cache* out, size_t max_bytes);
+int qk_cache_destroy(qk_cache c);
+/* GetColumn<T>: *hit = 1 when the block was cached.  QK_ERROR when bytes exceeds max_bytes
+ * ("Tried to cache block larger than maximum cache size"), QK_OUT_OF_MEMORY when nothing is
+ * left to evict ("Not enough space left in cache"); the reference throws std::length_error. */
+int qk_cache_get_column(qk_cache c, const char* database, const char* table_column, int32_t block_index,
+                        size_t bytes, int64_t load_size, int64_t load_offset, void** ptr, size_t* out_bytes,
+                        int32_t* hit);
+int qk_cache_contains(qk_cache c, const char* database, const char* table_column, int32_t block_index,
+                      int64_t load_size, int64_t load_offset, int32_t* out);
+/* Evict: the least recently used unlocked block; *evicted = 0 when there is none. */
+int qk_cache_evict(qk_cache c, int32_t* evicted);
+/* ClearCachedBlock: every cached load of (database, table_column, block_index). */
+int qk_cache_clear_block(qk_cache c, const char* database, const char* table_column, int32_t block_index);
+int qk_cache_set_current_block(qk_cache c, int64_t block_index);
+/* SetLockList: process-wide, like the reference's static list ("database.table.column"). */
+int qk_cache_set_lock_list(int32_t n, const char* const* columns);
+int qk_cache_stats(qk_cache c, size_t* used, int64_t* entries, int64_t* hits, int64_t* misses, int64_t* evictions);
 
 /* ======================================================================================
  * Filter (WHERE)
@@ -482,6 +522,16 @@ int qk_serve_release(qk_serve s, int64_t ticket);
 int qk_serve_replay(qk_serve s, int32_t n, const qk_batch_query* q, const double* arrival_ms,
                     double* latency_ms);
 int qk_serve_get_stats(qk_serve s, qk_serve_stats* out);
+
+/* The plain path under the same open-loop arrivals, for comparison with qk_serve_replay: one
+ * worker runs one request at a time to completion, in arrival order, as the reference's client
+ * worker runs its queries (TCPClientHandler.cpp:197-245, GpuSqlDispatcher.cpp:1107-1179).  Each
+ * request goes through the drop-in per-block path: the block's key column
+ * (qk_batch_key_column), the fused WHERE + GROUP BY of a GPUGroupBy table
+ * (qk_gb_process_block_where), then GetResults into device memory.  latency_ms[i] is the time
+ * from arrival_ms[i] (ms after the call starts) until request i's groups are in device memory. */
+int qk_serve_replay_plain(const qk_batch_columns* cols, int32_t n, const qk_batch_query* q,
+                          const double* arrival_ms, double* latency_ms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -10,10 +10,14 @@ from ._abi import (  # noqa: F401
     GREATER_EQUAL, INT, INT8, LESS, LESS_EQUAL, LOGICAL_AND, LOGICAL_OR, LONG, NOT_EQUAL,
     NP_OF, QK_ERROR, QK_HASH_TABLE_FULL, QK_SUCCESS, Predicate, QueryEngineError, load,
 )
-from .memory import DeviceBuffer, DeviceColumn, ensure_device, pack_nulls, unpack_nulls  # noqa: F401
+from .memory import (  # noqa: F401
+    BlockCache, DeviceBuffer, DeviceColumn, can_allocate, ensure_device, memory_limit, memory_stats, pack_nulls,
+    set_memory_limit, set_memory_limit_percent, unpack_nulls,
+)
 from .ops import Const, GPUBinary, GPUGroupBy, GPUReconstruct, GroupByResult, where  # noqa: F401
 
 __all__ = [
     "DeviceBuffer", "DeviceColumn", "GPUBinary", "GPUGroupBy", "GPUReconstruct", "Const",
-    "where", "QueryEngineError", "load", "ensure_device",
+    "where", "QueryEngineError", "load", "ensure_device", "BlockCache", "set_memory_limit",
+    "set_memory_limit_percent", "memory_limit", "can_allocate", "memory_stats",
 ]

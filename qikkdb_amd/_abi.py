@@ -195,6 +195,19 @@ PROTOTYPES = {
     "qk_batch_key_column": [_vp, _vp, _i32, _i32, _i32, _vp, _vp],
     "qk_batch_resample": [_vp],
     "qk_batch_estimate_rows": [_vp, _i32, _P(BatchQuery), _P(_i64)],
+    "qk_mem_set_limit": [_sz],
+    "qk_mem_set_limit_percent": [_i32],
+    "qk_mem_get_limit": [_P(_sz)],
+    "qk_mem_can_allocate": [_sz, _P(_i32)],
+    "qk_cache_create": [_P(_vp), _sz],
+    "qk_cache_destroy": [_vp],
+    "qk_cache_get_column": [_vp, C.c_char_p, C.c_char_p, _i32, _sz, _i64, _i64, _P(_vp), _P(_sz), _P(_i32)],
+    "qk_cache_contains": [_vp, C.c_char_p, C.c_char_p, _i32, _i64, _i64, _P(_i32)],
+    "qk_cache_evict": [_vp, _P(_i32)],
+    "qk_cache_clear_block": [_vp, C.c_char_p, C.c_char_p, _i32],
+    "qk_cache_set_current_block": [_vp, _i64],
+    "qk_cache_set_lock_list": [_i32, _P(C.c_char_p)],
+    "qk_cache_stats": [_vp, _P(_sz), _P(_i64), _P(_i64), _P(_i64), _P(_i64)],
     "qk_serve_create": [_P(_vp), _P(BatchColumns), _P(ServeConfig)],
     "qk_serve_destroy": [_vp],
     "qk_serve_submit": [_vp, _P(BatchQuery), _P(_i64)],
@@ -204,6 +217,7 @@ PROTOTYPES = {
     "qk_serve_release": [_vp, _i64],
     "qk_serve_replay": [_vp, _i32, _P(BatchQuery), _P(C.c_double), _P(C.c_double)],
     "qk_serve_get_stats": [_vp, _P(ServeStats)],
+    "qk_serve_replay_plain": [_P(BatchColumns), _i32, _P(BatchQuery), _P(C.c_double), _P(C.c_double), _vp],
 }
 
 _lib = None

@@ -18,7 +18,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
-HIP_SOURCES = ["runtime.hip", "filter.hip", "reconstruct.hip", "groupby.hip", "twophase.hip", "batch.hip", "serve.hip"]
+HIP_SOURCES = ["runtime.hip", "filter.hip", "reconstruct.hip", "groupby.hip", "twophase.hip", "batch.hip", "serve.hip",
+               "cache.hip"]
 LIB = os.path.join(PKG, "libqikkdb_amd.so")
 DATAGEN_LIB = os.path.join(PKG, "libqk_datagen.so")
 ARCH = "gfx950"

@@ -43,6 +43,9 @@ int ensure_init();
 // ---- pooled allocation + per-stream scratch -------------------------------------------
 int pool_alloc(void** p, size_t bytes);
 int pool_free(void* p);
+// Whether pool_alloc(bytes) would succeed under the pool's limit (or, without one, the device's
+// free memory): a cached block of the size, or room under the limit.
+bool pool_can_allocate(size_t bytes);
 
 // Scratch owned by one stream: grows on demand, zero-filled when (re)allocated.  Holds the
 // look-back status words, tickets and counters of the scan-based kernels; concurrent use

@@ -71,6 +71,7 @@ struct Pool {
     std::multimap<size_t, void*> free_blocks;      // rounded size -> block
     std::unordered_map<void*, size_t> live;        // block -> rounded size
     size_t in_use = 0, cached = 0;
+    size_t limit = 0;                              // bytes held from the driver at most (0: none)
 };
 std::mutex g_pools_mu;
 std::map<int, Pool*> g_pools;
@@ -113,6 +114,13 @@ int pool_alloc(void** out, size_t bytes) {
         p.in_use += r;
         return QK_SUCCESS;
     }
+    if (p.limit) {
+        // the cap covers what the pool holds from the driver: blocks in use and cached ones
+        if (p.in_use + r > p.limit)
+            return fail(QK_OUT_OF_MEMORY, "device allocation of %zu bytes exceeds the memory limit of %zu bytes "
+                        "(%zu in use)", r, p.limit, p.in_use);
+        if (p.in_use + p.cached + r > p.limit) release_locked(p);
+    }
     void* ptr = nullptr;
     hipError_t e = hipMalloc(&ptr, r);
     if (e != hipSuccess) {
@@ -144,6 +152,17 @@ int pool_free(void* ptr) {
     p.free_blocks.emplace(r, ptr);
     p.cached += r;
     return QK_SUCCESS;
+}
+
+bool pool_can_allocate(size_t bytes) {
+    Pool& p = pool_for_current();
+    const size_t r = round_size(bytes);
+    std::lock_guard<std::mutex> lk(p.mu);
+    if (p.free_blocks.count(r)) return true;
+    if (p.limit) return p.in_use + r <= p.limit;
+    size_t free_b = 0, total = 0;
+    if (hipMemGetInfo(&free_b, &total) != hipSuccess) return false;
+    return free_b + p.cached >= r;
 }
 
 // ---- per-stream scratch -----------------------------------------------------------------
@@ -235,6 +254,40 @@ int qk_mem_release_cached(void) {
     Pool& p = pool_for_current();
     std::lock_guard<std::mutex> lk(p.mu);
     return release_locked(p);
+}
+
+int qk_mem_set_limit(size_t bytes) {
+    Pool& p = pool_for_current();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.limit = bytes;
+    if (p.limit && p.in_use + p.cached > p.limit) release_locked(p);
+    return QK_SUCCESS;
+}
+
+int qk_mem_set_limit_percent(int32_t percent) {
+    if (percent < 1 || percent > 100) return fail(QK_ERROR, "memory limit: percent must be in 1..100, got %d", percent);
+    size_t free_b = 0, total = 0;
+    QK_HIP(hipMemGetInfo(&free_b, &total));
+    Pool& p = pool_for_current();
+    std::lock_guard<std::mutex> lk(p.mu);
+    // a share of what is free now plus what the pool already holds
+    p.limit = (free_b + p.in_use + p.cached) / 100 * (size_t)percent;
+    if (p.in_use + p.cached > p.limit) release_locked(p);
+    return QK_SUCCESS;
+}
+
+int qk_mem_get_limit(size_t* bytes) {
+    if (!bytes) return fail(QK_ERROR, "qk_mem_get_limit: null output");
+    Pool& p = pool_for_current();
+    std::lock_guard<std::mutex> lk(p.mu);
+    *bytes = p.limit;
+    return QK_SUCCESS;
+}
+
+int qk_mem_can_allocate(size_t bytes, int32_t* out) {
+    if (!out) return fail(QK_ERROR, "qk_mem_can_allocate: null output");
+    *out = pool_can_allocate(bytes) ? 1 : 0;
+    return QK_SUCCESS;
 }
 
 int qk_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream) {

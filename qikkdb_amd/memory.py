@@ -126,3 +126,95 @@ class DeviceColumn:
 
     def to_numpy(self) -> np.ndarray:
         return self.data.to_numpy(_abi.NP_OF[self.qtype], self.n)
+
+
+# ---- pool cap and block cache ---------------------------------------------------------------
+
+def set_memory_limit(nbytes: int) -> None:
+    """Cap the device memory the pool holds (0 = none): qk_mem_set_limit, the counterpart of
+    the reference allocator's GPUMemoryUsagePercent (CudaMemAllocator.cu:109-152)."""
+    ensure_device()
+    call("qk_mem_set_limit", int(nbytes))
+
+
+def set_memory_limit_percent(percent: int) -> None:
+    ensure_device()
+    call("qk_mem_set_limit_percent", int(percent))
+
+
+def memory_limit() -> int:
+    out = C.c_size_t()
+    call("qk_mem_get_limit", C.byref(out))
+    return out.value
+
+
+def can_allocate(nbytes: int) -> bool:
+    """CudaMemAllocator::CanAllocate (CudaMemAllocator.h:57)."""
+    out = C.c_int32()
+    call("qk_mem_can_allocate", int(nbytes), C.byref(out))
+    return bool(out.value)
+
+
+def memory_stats() -> tuple[int, int]:
+    """(bytes in use, bytes cached) of the pool."""
+    u, c = C.c_size_t(), C.c_size_t()
+    call("qk_mem_stats", C.byref(u), C.byref(c))
+    return u.value, c.value
+
+
+class BlockCache:
+    """LRU cache of device column blocks (qk_cache_*): the reference's GPUMemoryCache
+    (QueryEngine/GPUMemoryCache.h:10-141).  get_column returns (DeviceBuffer not owned by the
+    caller, bytes, hit)."""
+
+    def __init__(self, max_bytes: int):
+        ensure_device()
+        self.h = C.c_void_p()
+        call("qk_cache_create", C.byref(self.h), int(max_bytes))
+
+    def close(self) -> None:
+        if self.h:
+            call("qk_cache_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_column(self, database: str, table_column: str, block: int, nbytes: int,
+                   load_size: int = -1, load_offset: int = 0) -> tuple[DeviceBuffer, int, bool]:
+        p, got, hit = C.c_void_p(), C.c_size_t(), C.c_int32()
+        call("qk_cache_get_column", self.h, database.encode(), table_column.encode(), block, int(nbytes),
+             load_size, load_offset, C.byref(p), C.byref(got), C.byref(hit))
+        return DeviceBuffer(got.value, ptr=p.value, owned=False), got.value, bool(hit.value)
+
+    def contains(self, database: str, table_column: str, block: int, load_size: int = -1,
+                 load_offset: int = 0) -> bool:
+        out = C.c_int32()
+        call("qk_cache_contains", self.h, database.encode(), table_column.encode(), block, load_size,
+             load_offset, C.byref(out))
+        return bool(out.value)
+
+    def evict(self) -> bool:
+        out = C.c_int32()
+        call("qk_cache_evict", self.h, C.byref(out))
+        return bool(out.value)
+
+    def clear_block(self, database: str, table_column: str, block: int) -> None:
+        call("qk_cache_clear_block", self.h, database.encode(), table_column.encode(), block)
+
+    def set_current_block(self, block: int) -> None:
+        call("qk_cache_set_current_block", self.h, block)
+
+    @staticmethod
+    def set_lock_list(columns) -> None:
+        arr = (C.c_char_p * max(1, len(columns)))(*[c.encode() for c in columns])
+        call("qk_cache_set_lock_list", len(columns), arr)
+
+    def stats(self) -> dict:
+        used, ent, hits, miss, ev = C.c_size_t(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        call("qk_cache_stats", self.h, C.byref(used), C.byref(ent), C.byref(hits), C.byref(miss), C.byref(ev))
+        return {"used": used.value, "entries": ent.value, "hits": hits.value, "misses": miss.value,
+                "evictions": ev.value}

@@ -25,6 +25,7 @@
 
 #include "qikkdb/GPUBinary.h"
 #include "qikkdb/GPUGroupBy.h"
+#include "qikkdb/GPUMemoryCache.h"
 #include "qikkdb/GPUReconstruct.h"
 
 namespace {
@@ -72,6 +73,32 @@ struct Block {
         GPUMemory::free(data);
         GPUMemory::free(nulls);
     }
+};
+
+// A block of a column loaded through the device's block cache, as GpuSqlDispatcher::LoadCol
+// does: GetColumn<T>, and on a miss the block (and its null mask, cached as
+// "<column>_nullMask") is copied from the host.  The cache owns the memory.
+struct CachedBlock {
+    int32_t* data = nullptr;
+    nullmask_t* nulls = nullptr;
+    CachedBlock(GPUMemoryCache& cache, const std::string& column, const int32_t* host, const uint8_t* nbytes,
+                int32_t blockIndex, int32_t m, int64_t& hits, int64_t& misses) {
+        auto d = cache.GetColumn<int32_t>("db", column, blockIndex, m, -1, 0);
+        data = std::get<0>(d);
+        if (std::get<2>(d)) hits++; else misses++;
+        if (!std::get<2>(d)) GPUMemory::copyHostToDevice(data, host, m);
+        if (nbytes) {
+            std::vector<nullmask_t> w = pack(nbytes, m);
+            auto nm = cache.GetColumn<nullmask_t>("db", column + "_nullMask", blockIndex, w.size(), -1, 0);
+            nulls = std::get<0>(nm);
+            if (std::get<2>(nm)) hits++; else misses++;
+            if (!std::get<2>(nm)) GPUMemory::copyHostToDevice(nulls, w.data(), w.size());
+        }
+    }
+};
+
+struct CacheCounts {
+    int64_t hits = 0, misses = 0;
 };
 
 // The part of the reference dispatcher that the GROUP BY helpers touch.
@@ -170,7 +197,8 @@ struct Columns {
 // The dispatcher's per-block sequence for one aggregate: WHERE mask, compaction of every
 // column the GROUP BY reads (with null masks), ProcessBlock on the block's dispatcher thread.
 template <typename Helper, bool MULTI>
-void run_query(const char* agg, const Columns& c, int32_t block, int32_t threshold) {
+void run_query(const char* agg, const Columns& c, int32_t block, int32_t threshold, GPUMemoryCache& cache,
+               CacheCounts& cc) {
     Dispatcher d;
     std::vector<DataType> kt = {COLUMN_INT, COLUMN_INT};
     for (int t = 0; t < 2; t++) {
@@ -183,8 +211,11 @@ void run_query(const char* agg, const Columns& c, int32_t block, int32_t thresho
     for (int32_t s = 0; s < c.n; s += block, blockIndex++) {
         const int32_t m = std::min(block, c.n - s);
         d.dispatcherThreadId_ = blockIndex % 2;
-        Block a(c.a.data() + s, c.an.data() + s, m), b(c.b.data() + s, c.bn.data() + s, m);
-        Block v(c.v.data() + s, c.vn.data() + s, m), f(c.f.data() + s, nullptr, m);
+        cache.SetCurrentBlockIndex(blockIndex);
+        CachedBlock a(cache, "t.a", c.a.data() + s, c.an.data() + s, blockIndex, m, cc.hits, cc.misses);
+        CachedBlock b(cache, "t.b", c.b.data() + s, c.bn.data() + s, blockIndex, m, cc.hits, cc.misses);
+        CachedBlock v(cache, "t.v", c.v.data() + s, c.vn.data() + s, blockIndex, m, cc.hits, cc.misses);
+        CachedBlock f(cache, "t.f", c.f.data() + s, nullptr, blockIndex, m, cc.hits, cc.misses);
         int8_t* mask = nullptr;
         GPUMemory::alloc(&mask, m);
         GPUBinary<FilterConditions::greater, int8_t, int32_t*, int32_t>::Binary(mask, f.data, threshold, m);
@@ -381,16 +412,25 @@ int main(int argc, char** argv) {
     try {
         qikkdb::check(qk_init(0));
         using namespace AggregationFunctions;
-        run_query<GroupByHelper<min, int32_t, int32_t, int32_t>, false>("min", c, block, threshold);
-        run_query<GroupByHelper<max, int32_t, int32_t, int32_t>, false>("max", c, block, threshold);
-        run_query<GroupByHelper<sum, int32_t, int32_t, int32_t>, false>("sum", c, block, threshold);
-        run_query<GroupByHelper<avg, int32_t, int32_t, int32_t>, false>("avg", c, block, threshold);
-        run_query<GroupByHelper<count, int64_t, int32_t, int32_t>, false>("count", c, block, threshold);
-        run_query<MultiGroupByHelper<min, int32_t, int32_t>, true>("min", c, block, threshold);
-        run_query<MultiGroupByHelper<max, int32_t, int32_t>, true>("max", c, block, threshold);
-        run_query<MultiGroupByHelper<sum, int32_t, int32_t>, true>("sum", c, block, threshold);
-        run_query<MultiGroupByHelper<avg, int32_t, int32_t>, true>("avg", c, block, threshold);
-        run_query<MultiGroupByHelper<count, int64_t, int32_t>, true>("count", c, block, threshold);
+        {
+            // every block of the four columns and three null masks fits: the first query loads
+            // them, the other nine find them cached
+            const int32_t nblocks = (c.n + block - 1) / block;
+            GPUMemoryCache cache(0, (size_t)c.n * 4 * 4 + (size_t)nblocks * 3 * 8 * (block / 64 + 1));
+            CacheCounts cc;
+            run_query<GroupByHelper<min, int32_t, int32_t, int32_t>, false>("min", c, block, threshold, cache, cc);
+            run_query<GroupByHelper<max, int32_t, int32_t, int32_t>, false>("max", c, block, threshold, cache, cc);
+            run_query<GroupByHelper<sum, int32_t, int32_t, int32_t>, false>("sum", c, block, threshold, cache, cc);
+            run_query<GroupByHelper<avg, int32_t, int32_t, int32_t>, false>("avg", c, block, threshold, cache, cc);
+            run_query<GroupByHelper<count, int64_t, int32_t, int32_t>, false>("count", c, block, threshold, cache, cc);
+            run_query<MultiGroupByHelper<min, int32_t, int32_t>, true>("min", c, block, threshold, cache, cc);
+            run_query<MultiGroupByHelper<max, int32_t, int32_t>, true>("max", c, block, threshold, cache, cc);
+            run_query<MultiGroupByHelper<sum, int32_t, int32_t>, true>("sum", c, block, threshold, cache, cc);
+            run_query<MultiGroupByHelper<avg, int32_t, int32_t>, true>("avg", c, block, threshold, cache, cc);
+            run_query<MultiGroupByHelper<count, int64_t, int32_t>, true>("count", c, block, threshold, cache, cc);
+            if (cc.misses != (int64_t)nblocks * 7 || cc.hits != 9 * cc.misses) fail_check("block cache hits");
+            std::printf("CHECK block cache ok\n");
+        }
         check_reconstruct(c, threshold);
         check_seeded_and_raw(c);
         // an empty tables vector is refused as in the reference

@@ -107,7 +107,7 @@ def test_dispatcher_shaped_caller(tmp_path):
         assert key not in m, f"duplicate group {key} in {p[0]} {agg}"
         m[key] = val
     assert checks == {"reconstructCol", "GenerateIndexes", "PrefixSum", "Sum", "CompressNullMask", "seeded",
-                      "ReconstructRawNumbers", "empty"}
+                      "ReconstructRawNumbers", "empty", "block"}
     preds = [O.pred(0, f, threshold, O.INT)]
     for form, keys, kn in (("S", [a], [_pack(an)]), ("M", [a, b], [_pack(an), _pack(bn)])):
         for agg, code in AGG_CODES.items():
