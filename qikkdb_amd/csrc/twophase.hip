@@ -144,42 +144,64 @@ __global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const
     __syncthreads();
     const int64_t per = (nsamples + gridDim.x - 1) / gridDim.x;
     const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min(nsamples, i0 + per);
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const int64_t r = i * stride;
-        if (r >= n) break;
-        typename std::conditional<VTraits<VT>::has, VT, int>::type v{};
-        if constexpr (VTraits<VT>::has) v = vals[r];
-        const bool vn = vnull && null_bit(vnull, r);
-        int64_t cv = 0;
-        bool cn = false;
-        if (MODE == 4) {
-            cv = range_col(preds, r);
-            cn = preds.rnull && null_bit(preds.rnull, r);
+    // kBatch samples per thread at a time, their loads issued before any is used (the samples
+    // are rows far apart, so each is its own memory round trip)
+    constexpr int kBatch = 4;
+    using VV = typename std::conditional<VTraits<VT>::has, VT, int>::type;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kBatch * (int64_t)blockDim.x) {
+        VV v[kBatch];
+        bool vn[kBatch], cn[kBatch], kn[kBatch], ok[kBatch];
+        int64_t cv[kBatch], rr[kBatch];
+        uint64_t k[kBatch][1];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            // runs of 16 consecutive rows, `stride` sample rows apart, so that the loads of a
+            // wavefront touch few lines
+            const int64_t iu = i + (int64_t)u * blockDim.x;
+            const int64_t r = (iu >> 4) * (stride << 4) + (iu & 15);
+            ok[u] = iu < i1 && r < n;
+            rr[u] = ok[u] ? r : 0;
+            v[u] = VV{};
+            if constexpr (VTraits<VT>::has) v[u] = vals[rr[u]];
+            vn[u] = vnull && null_bit(vnull, rr[u]);
+            cv[u] = 0;
+            cn[u] = false;
+            if (MODE == 4) {
+                cv[u] = range_col(preds, rr[u]);
+                cn[u] = preds.rnull && null_bit(preds.rnull, rr[u]);
+            }
+            load_key<1>(ks, kc, rr[u], k[u], kn[u]);
         }
-        if (!row_passes<VT, MODE>(preds, vpred, mask, r, v, vn, cv, cn)) continue;
-        uint64_t k[1];
-        bool kn;
-        load_key<1>(ks, kc, r, k, kn);
-        if (kn || k[0] == kEmpty) continue;  // those keys have side slots anyway
-        uint32_t s = (uint32_t)(hash64(k[0]) >> 53) & (kSampleLds - 1);
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+        if (!ok[u] || !row_passes<VT, MODE>(preds, vpred, mask, rr[u], v[u], vn[u], cv[u], cn[u])) continue;
+        if (kn[u] || k[u][0] == kEmpty) continue;  // those keys have side slots anyway
+        const uint64_t key = k[u][0];
+        uint32_t s = (uint32_t)(hash64(key) >> 53) & (kSampleLds - 1);
         bool done = false;
         for (int p = 0; p < 16 && !done; p++) {
             unsigned long long old = atomicCAS((unsigned long long*)&lk[s], (unsigned long long)kEmpty,
-                                               (unsigned long long)k[0]);
-            if (old == kEmpty || old == k[0]) {
+                                               (unsigned long long)key);
+            if (old == kEmpty || old == key) {
                 atomicAdd(&lc[s], 1u);
                 done = true;
             }
             s = (s + 1) & (kSampleLds - 1);
         }
-        if (!done) sample_add(skeys, scnt, smask, k[0], 1u);
+        if (!done) sample_add(skeys, scnt, smask, key, 1u);
+        }
     }
     __syncthreads();
     for (int s = threadIdx.x; s < kSampleLds; s += blockDim.x)
         if (lc[s]) sample_add(skeys, scnt, smask, lk[s], lc[s]);
 }
 
-__global__ void __launch_bounds__(256) k_tp_hist(const uint64_t* __restrict__ skeys,
+// Histogram of the sample counts (clamped to kHistBins - 1): per workgroup in LDS, then one
+// device atomic per non-empty bin and workgroup, so few large workgroups keep the device
+// atomics on the hot low bins few.
+constexpr int kHistBlock = 1024;
+constexpr int kHistGrid = 64;
+__global__ void __launch_bounds__(kHistBlock) k_tp_hist(const uint64_t* __restrict__ skeys,
                                                  const uint32_t* __restrict__ scnt, uint64_t slots,
                                                  uint32_t* hist) {
     __shared__ uint32_t lh[kHistBins];
@@ -212,36 +234,87 @@ __global__ void __launch_bounds__(kHistBins) k_tp_threshold(const uint32_t* hist
     }
 }
 
-__global__ void __launch_bounds__(256) k_tp_collect(const uint64_t* __restrict__ skeys,
-                                                    const uint32_t* __restrict__ scnt, uint64_t slots,
-                                                    int32_t kmax, int32_t* meta, uint64_t* cand,
-                                                    uint32_t* cand_cnt) {
+// Keys seen at least meta[1] times, appended to cand / cand_cnt (meta[0] counts them).  Each
+// workgroup collects the candidates of its range of slots in LDS and reserves their place with
+// one atomic, so the counter sees one atomic per workgroup (at most kmax candidates in all).
+constexpr int kCollectBlock = 1024;
+constexpr int kCollectGrid = 64;
+__global__ void __launch_bounds__(kCollectBlock) k_tp_collect(const uint64_t* __restrict__ skeys,
+                                                              const uint32_t* __restrict__ scnt, uint64_t slots,
+                                                              int32_t kmax, int32_t* meta, uint64_t* cand,
+                                                              uint32_t* cand_cnt) {
+    __shared__ uint64_t lkey[4096];
+    __shared__ uint32_t lcnt[4096];
+    __shared__ int32_t ln, lbase;
     const uint32_t t = (uint32_t)meta[1];
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < slots; s += stride) {
-        if (skeys[s] != kEmpty && scnt[s] >= t) {
-            int32_t i = atomicAdd(meta, 1);
-            if (i < kmax) {
-                cand[i] = skeys[s];
-                cand_cnt[i] = scnt[s];
+    if (threadIdx.x == 0) ln = 0;
+    __syncthreads();
+    const uint64_t per = (slots + gridDim.x - 1) / gridDim.x;
+    const uint64_t s0 = (uint64_t)blockIdx.x * per, s1 = min(slots, s0 + per);
+    for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        const uint64_t key = skeys[s];
+        const uint32_t c = scnt[s];
+        if (key != kEmpty && c >= t) {
+            const int32_t i = atomicAdd(&ln, 1);
+            if (i < 4096) {
+                lkey[i] = key;
+                lcnt[i] = c;
             }
+        }
+    }
+    __syncthreads();
+    const int32_t n = min(ln, min(kmax, 4096));
+    if (threadIdx.x == 0) lbase = n ? atomicAdd(meta, n) : 0;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int32_t at = lbase + i;
+        if (at < kmax) {
+            cand[at] = lkey[i];
+            cand_cnt[at] = lcnt[i];
         }
     }
 }
 
-// One workgroup: order the heavy candidates by partition; meta[2 + p] = first index of p.
-__global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ cand,
-                                                  const uint32_t* __restrict__ cand_cnt, int32_t kmax,
-                                                  int32_t P, int32_t part_shift, int32_t* meta,
-                                                  uint64_t* heavy, uint32_t* heavy_cnt) {
-    __shared__ int32_t cnt[1024];
-    __shared__ int32_t cur[1024];
+// ---- heavy set order and heavy table image ---------------------------------------------------
+
+// The bucket of a key in the heavy table: a multiplicative hash of the key's two halves folded
+// together (Knuth's golden-ratio multiplier; the high bits of the product pick the bucket).
+// The table only needs keys spread over the buckets, so this stands in for the full 64-bit
+// hash on the path every row takes.
+__device__ __forceinline__ uint32_t bucket_of(uint64_t k, uint32_t nb) {
+    const uint32_t f = ((uint32_t)k ^ (uint32_t)(k >> 32)) * 0x9e3779b1u;
+    return __umulhi(f, nb);
+}
+
+// One workgroup, once per heavy set:
+// 1. orders the heavy candidates by partition (heavy[], heavy_cnt[]; meta[2 + p] = first
+//    index of partition p, meta[2 + P] = K, meta[0] = K);
+// 2. places each heavy key in a free slot of its bucket of an H-slot table, the most frequent
+//    keys first (one round per power of two of the sample count), so that the keys carrying
+//    most rows always find a slot.  A key that finds its bucket full stays cold (its rows then
+//    go through its partition like any other key's).  The table's keys go to image[H] (kEmpty
+//    in free slots) and each heavy key's slot to slot[k] (-1 when not placed); every phase-1
+//    workgroup starts from this image.
+// Dynamic LDS: H keys of the table + kmax keys and counts of the ordered set.
+__global__ void __launch_bounds__(1024) k_tp_order(const uint64_t* __restrict__ cand,
+                                                   const uint32_t* __restrict__ cand_cnt, int32_t kmax,
+                                                   int32_t P, int32_t part_shift, int32_t* meta,
+                                                   uint64_t* __restrict__ heavy, uint32_t* __restrict__ heavy_cnt,
+                                                   int32_t H, uint64_t* __restrict__ image,
+                                                   int32_t* __restrict__ slot) {
+    extern __shared__ __align__(16) uint64_t lds[];
+    __shared__ int32_t cnt[kMaxRecParts];
+    __shared__ int32_t cur[kMaxRecParts];
     __shared__ int32_t wsum[16];
-    const int t = threadIdx.x;
+    uint64_t* lkey = lds;                                       // [H]
+    uint64_t* okey = lds + H;                                   // [kmax] ordered heavy keys
+    uint32_t* ocnt = reinterpret_cast<uint32_t*>(okey + kmax);  // [kmax] their sample counts
+    const int t = threadIdx.x, bd = blockDim.x;
     const int32_t K = min(meta[0], kmax);
-    for (int p = t; p < P; p += blockDim.x) cnt[p] = 0;
+    for (int p = t; p < P; p += bd) cnt[p] = 0;
+    for (int i = t; i < H; i += bd) lkey[i] = kEmpty;
     __syncthreads();
-    for (int i = t; i < K; i += blockDim.x) {
+    for (int i = t; i < K; i += bd) {
         const uint64_t h = hash64(cand[i]);
         const int p = part_shift >= 64 ? 0 : (int)(h >> part_shift);
         atomicAdd(&cnt[p], 1);
@@ -258,43 +331,19 @@ __global__ void __launch_bounds__(1024) k_tp_sort(const uint64_t* __restrict__ c
         meta[0] = K;
     }
     __syncthreads();
-    for (int i = t; i < K; i += blockDim.x) {
-        const uint64_t h = hash64(cand[i]);
+    for (int i = t; i < K; i += bd) {
+        const uint64_t key = cand[i];
+        const uint64_t h = hash64(key);
         const int p = part_shift >= 64 ? 0 : (int)(h >> part_shift);
         const int32_t at = atomicAdd(&cur[p], 1);
-        heavy[at] = cand[i];
+        heavy[at] = key;
         heavy_cnt[at] = cand_cnt[i];
+        okey[at] = key;
+        ocnt[at] = cand_cnt[i];
     }
-}
-
-// ---- heavy table image -------------------------------------------------------------------
-
-// The bucket of a key in the heavy table: a multiplicative hash of the key's two halves folded
-// together (Knuth's golden-ratio multiplier; the high bits of the product pick the bucket).
-// The table only needs keys spread over the buckets, so this stands in for the full 64-bit
-// hash on the path every row takes.
-__device__ __forceinline__ uint32_t bucket_of(uint64_t k, uint32_t nb) {
-    const uint32_t f = ((uint32_t)k ^ (uint32_t)(k >> 32)) * 0x9e3779b1u;
-    return __umulhi(f, nb);
-}
-
-// One workgroup, once per heavy set: places each heavy key in a free slot of its bucket of an
-// H-slot table, the most frequent keys first (one round per power of two of the sample count),
-// so that the keys carrying most rows always find a slot.  A key that finds its bucket full
-// stays cold (its rows then go through its partition like any other key's).  The table's keys
-// go to image[H] (kEmpty in free slots) and each heavy key's slot to slot[k] (-1 when not
-// placed); every phase-1 workgroup starts from this image instead of placing the keys itself.
-__global__ void __launch_bounds__(1024) k_tp_place(const uint64_t* __restrict__ heavy,
-                                                   const uint32_t* __restrict__ heavy_cnt,
-                                                   const int32_t* __restrict__ meta, int32_t kmax, int32_t H,
-                                                   uint64_t* __restrict__ image, int32_t* __restrict__ slot) {
-    extern __shared__ __align__(16) uint64_t lds[];
-    uint64_t* lkey = lds;
-    const int tid = threadIdx.x, bd = blockDim.x;
-    const uint32_t nb = (uint32_t)(H / kBucket);
-    for (int i = tid; i < H; i += bd) lkey[i] = kEmpty;
     __syncthreads();
-    const int32_t K = min(meta[0], kmax);
+
+    const uint32_t nb = (uint32_t)(H / kBucket);
     // kmax <= 4 * blockDim, so each thread holds at most kPlace heavy keys
     constexpr int kPlace = 4;
     static_assert(kPlace * 1024 >= 4096, "heavy keys per thread");
@@ -302,12 +351,12 @@ __global__ void __launch_bounds__(1024) k_tp_place(const uint64_t* __restrict__ 
     int pcls[kPlace];
 #pragma unroll
     for (int i = 0; i < kPlace; i++) {
-        const int k = tid + i * bd;
+        const int k = t + i * bd;
         pcls[i] = -1;
         pkey[i] = 0;
         if (k < K) {
-            pkey[i] = heavy[k];
-            pcls[i] = 31 - __builtin_clz(max(heavy_cnt[k], 1u));
+            pkey[i] = okey[k];
+            pcls[i] = 31 - __builtin_clz(max(ocnt[k], 1u));
         }
     }
     for (int cls = 31; cls >= 0; cls--) {
@@ -325,17 +374,17 @@ __global__ void __launch_bounds__(1024) k_tp_place(const uint64_t* __restrict__ 
                                                          (unsigned long long)kEmpty, (unsigned long long)pkey[i]);
                 if (old == kEmpty) placed = (int)(bb * kBucket + j);
             }
-            slot[tid + i * bd] = placed;
+            slot[t + i * bd] = placed;
         }
     }
     __syncthreads();
-    for (int i = tid; i < H; i += bd) image[i] = lkey[i];
+    for (int i = t; i < H; i += bd) image[i] = lkey[i];
 }
 
 // ---- phase 1 -------------------------------------------------------------------------------
 
 struct P1Args {
-    const uint64_t* image;      // [H] keys of the heavy table (k_tp_place)
+    const uint64_t* image;      // [H] keys of the heavy table (k_tp_order)
     const int32_t* slot;        // [kmax] slot of each heavy key in it, -1 if not placed
     const int32_t* meta;
     int32_t kmax, H, P, cap;
@@ -1518,8 +1567,9 @@ int twophase_prepare(TwoPhaseBufs& b, int32_t parts, uint32_t fields, int32_t va
     QK_HIP(hipMemsetAsync(b.heavy_meta, 0, (size_t)(parts + 8) * 4, s));
     QK_HIP(hipMemsetAsync(b.zero_word, 0, 64, s));
     // the image of the empty heavy set
-    k_tp_place<<<1, 1024, (size_t)b.H1 * 8, s>>>(b.heavy_keys, b.heavy_cnt, b.heavy_meta, b.kmax, b.H1,
-                                                 b.p1_image, b.p1_slot);
+    k_tp_order<<<1, 1024, (size_t)b.H1 * 8 + (size_t)b.kmax * 12, s>>>(
+        b.cand, b.cand_cnt, b.kmax, b.P, rec_shift_of(b.P), b.heavy_meta, b.heavy_keys, b.heavy_cnt, b.H1,
+        b.p1_image, b.p1_slot);
     QK_HIP(hipGetLastError());
     b.heavy_ready = false;
     return QK_SUCCESS;
@@ -1547,14 +1597,13 @@ int twophase_sample(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
         }
     }
     if (st != QK_SUCCESS) return st;
-    k_tp_hist<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots, b.hist);
+    k_tp_hist<<<kHistGrid, kHistBlock, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots, b.hist);
     k_tp_threshold<<<1, kHistBins, 0, s>>>(b.hist, b.kmax, b.heavy_meta);
-    k_tp_collect<<<grid_cap(sslots, 256, cus * 4), 256, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots,
+    k_tp_collect<<<kCollectGrid, kCollectBlock, 0, s>>>(b.sample_keys, b.sample_cnt, (uint64_t)sslots,
                                                                b.kmax, b.heavy_meta, b.cand, b.cand_cnt);
-    k_tp_sort<<<1, 1024, 0, s>>>(b.cand, b.cand_cnt, b.kmax, b.P, rec_shift_of(b.P), b.heavy_meta, b.heavy_keys,
-                                 b.heavy_cnt);
-    k_tp_place<<<1, 1024, (size_t)b.H1 * 8, s>>>(b.heavy_keys, b.heavy_cnt, b.heavy_meta, b.kmax, b.H1,
-                                                 b.p1_image, b.p1_slot);
+    k_tp_order<<<1, 1024, (size_t)b.H1 * 8 + (size_t)b.kmax * 12, s>>>(
+        b.cand, b.cand_cnt, b.kmax, b.P, rec_shift_of(b.P), b.heavy_meta, b.heavy_keys, b.heavy_cnt, b.H1,
+        b.p1_image, b.p1_slot);
     QK_HIP(hipGetLastError());
     b.heavy_ready = true;
     return QK_SUCCESS;
