@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -260,5 +261,61 @@ public:
         if (tables.size() <= 0) CheckQueryEngineError(GPU_EXTENSION_ERROR, "Number of tables have to be at least 1.");
         qikkdb::merge_tables(this, table_.get(), tables);
         GetResults(outKeysVector, outValues, outDataElementCount, outKeysNullMasksVector, outValuesNullMask);
+    }
+};
+
+// GROUP BY over one string key column (GPUGroupByString.cuh:169-785): keys arrive and leave as
+// GPUString columns; every other rule is the numeric table's.
+template <typename AGG, typename O, typename V>
+class GPUGroupBy<AGG, O, std::string, V> : public IGroupBy {
+    static constexpr bool USE_VALUES = std::is_same<AGG, AggregationFunctions::min>::value ||
+                                       std::is_same<AGG, AggregationFunctions::max>::value ||
+                                       std::is_same<AGG, AggregationFunctions::sum>::value ||
+                                       std::is_same<AGG, AggregationFunctions::avg>::value;
+    qk_gbs h_ = nullptr;
+
+public:
+    GPUGroupBy(const int32_t maxHashCount, const int32_t keySizeMultiplier) {
+        qikkdb::check_output_type<AGG, O, V>();
+        const int32_t agg = AGG::code, ot = qikkdb::TypeCode<O>::value;
+        qikkdb::check(qk_gbs_create(&h_, qikkdb::TypeCode<V>::value, 1, &agg, &ot, maxHashCount, keySizeMultiplier, 0));
+    }
+    GPUGroupBy(const GPUGroupBy&) = delete;
+    GPUGroupBy& operator=(const GPUGroupBy&) = delete;
+    ~GPUGroupBy() {
+        if (h_) qk_gbs_destroy(h_);
+    }
+
+    void ProcessBlock(GPUMemory::GPUString inKeys, V* inValues, int32_t dataElementCount,
+                      nullmask_t* inKeysNullMask = nullptr, nullmask_t* inValuesNullMask = nullptr) {
+        if (dataElementCount <= 0) return;
+        qikkdb::check(qk_gbs_process_block(h_, inKeys.allChars, inKeys.stringIndices, inKeysNullMask,
+                                           USE_VALUES ? inValues : nullptr, inValuesNullMask, dataElementCount,
+                                           nullptr));
+    }
+
+    void GetResults(GPUMemory::GPUString* outKeys, O** outValues, int32_t* outDataElementCount,
+                    nullmask_t** outKeysNullMask = nullptr, nullmask_t** outValuesNullMask = nullptr) {
+        void* vals[1] = {nullptr};
+        uint64_t* vnulls[1] = {nullptr};
+        uint64_t* kn = nullptr;
+        qikkdb::check(qk_gbs_get_results(h_, &outKeys->allChars, &outKeys->stringIndices, &kn, vals,
+                                         outValuesNullMask ? vnulls : nullptr, outDataElementCount, nullptr));
+        *outValues = static_cast<O*>(vals[0]);
+        if (outKeysNullMask) *outKeysNullMask = kn;
+        else if (kn) qk_free(kn);
+        if (outValuesNullMask) *outValuesNullMask = vnulls[0];
+    }
+
+    // The dispatcher's GetResults over its tables; one table in this one-GPU build.
+    void GetResults(GPUMemory::GPUString* outKeys, O** outValues, int32_t* outDataElementCount,
+                    std::vector<std::unique_ptr<IGroupBy>>& tables, nullmask_t** outKeysNullMask = nullptr,
+                    nullmask_t** outValuesNullMask = nullptr) {
+        if (tables.size() <= 0) throw std::invalid_argument("Number of tables have to be at least 1.");
+        for (auto& t : tables)
+            if (t && t.get() != this)
+                throw query_engine_error(GPU_EXTENSION_ERROR,
+                                         "GetResults: string keys merge one table per GPU; this is a one-GPU build");
+        GetResults(outKeys, outValues, outDataElementCount, outKeysNullMask, outValuesNullMask);
     }
 };

@@ -4,10 +4,18 @@
 #pragma once
 
 #include <cstddef>
+#include <cstdint>
 
 #include "GPUError.h"
 
 namespace GPUMemory {
+
+// A device string column (GPUMemory.cuh:70-77): every string's bytes in allChars, and
+// stringIndices[i] = the end offset of string i ("start indices shifted by one string").
+struct GPUString {
+    char* allChars = nullptr;
+    int64_t* stringIndices = nullptr;
+};
 
 template <typename T>
 void alloc(T** p_Block, size_t dataElementCount) {
@@ -39,6 +47,11 @@ void copyDeviceToHost(T* p_BlockHost, const T* p_BlockDevice, size_t dataElement
 
 inline void free(void* p_block) {
     if (p_block) qikkdb::check(qk_free(p_block));
+}
+
+inline void free(GPUString stringCol) {
+    free(stringCol.allChars);
+    free(stringCol.stringIndices);
 }
 
 }  // namespace GPUMemory

@@ -346,6 +346,31 @@ int qk_gb_seed_keys(qk_groupby h, const void* keys, int32_t n, int32_t first_key
  * merged on the device, so AVG stays exact.  Waits; QK_HASH_TABLE_FULL when dst fills up. */
 int qk_gb_merge(qk_groupby dst, qk_groupby src, void* stream);
 
+/* ---- GROUP BY over one string key column: GPUGroupBy<AGG,O,std::string,V>
+ * (GPUGroupByString.cuh:39-785).  A string column is the reference's GPUString
+ * (GPUMemory.cuh:70-77): chars = every string's bytes concatenated, indices[i] = end offset of
+ * string i (string i is chars[indices[i-1] .. indices[i]), indices[-1] = 0), both device
+ * pointers.  Aggregates, value types, NULL rules, capacity and errors are those of qk_gb_*
+ * (the table holds max_hash_count * key_size_multiplier distinct non-NULL strings).  Every
+ * ProcessBlock waits (it reads the block's byte count and the dictionary's state back). */
+typedef struct qk_gbs_s* qk_gbs;
+int qk_gbs_create(qk_gbs* out, int32_t value_type, int32_t naggs, const int32_t* aggs, const int32_t* out_types,
+                  int32_t max_hash_count, int32_t key_size_multiplier, uint32_t flags);
+int qk_gbs_destroy(qk_gbs h);
+int qk_gbs_reset(qk_gbs h, void* stream);
+int qk_gbs_process_block(qk_gbs h, const char* chars, const int64_t* indices, const uint64_t* key_nulls,
+                         const void* values, const uint64_t* value_nulls, int32_t n, void* stream);
+/* As qk_gbs_process_block, for the rows that pass every predicate (a conjunctive WHERE). */
+int qk_gbs_process_block_where(qk_gbs h, const char* chars, const int64_t* indices, const uint64_t* key_nulls,
+                               const void* values, const uint64_t* value_nulls, int32_t n, int32_t npreds,
+                               const qk_predicate* preds, void* stream);
+/* One row per group: the keys as a GPUString (out_chars, out_indices; the NULL group's string
+ * is empty and its bit set in out_key_nulls), the aggregates as in qk_gb_get_results.  All
+ * outputs come from qk_malloc. */
+int qk_gbs_get_results(qk_gbs h, char** out_chars, int64_t** out_indices, uint64_t** out_key_nulls,
+                       void** out_values, uint64_t** out_value_nulls, int32_t* out_count, void* stream);
+int qk_gbs_group_count(qk_gbs h, int64_t* out, void* stream);
+
 /* Number of groups currently in the table (waits for the table's work). */
 int qk_gb_group_count(qk_groupby h, int64_t* out, void* stream);
 

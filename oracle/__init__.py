@@ -217,6 +217,34 @@ class GroupBy:
                 "values": values, "value_nulls": [v.astype(bool) for v in vnull]}
 
 
+class GroupByStrings:
+    """Oracle GROUP BY over one string key column (GPUGroupByString.cuh:39-785 semantics): each
+    distinct byte string is numbered in order of first appearance with a Python dict (exact
+    equality), and the numbers are grouped by the oracle table above as int64 keys; results()
+    maps them back.  A NULL key (key_nulls) is the NULL group."""
+
+    def __init__(self, aggs, value_type, out_types=None):
+        self.ids = {}
+        self.names = []
+        self.g = GroupBy(aggs, [LONG], value_type, out_types)
+
+    def process(self, strings, values=None, value_nulls=None, key_nulls=None, preds=()):
+        ids = np.empty(len(strings), np.int64)
+        for i, s in enumerate(strings):
+            b = s.encode() if isinstance(s, str) else bytes(s)
+            k = self.ids.get(b)
+            if k is None:
+                k = self.ids[b] = len(self.names)
+                self.names.append(b)
+            ids[i] = k
+        self.g.process([ids], values, value_nulls, [key_nulls] if key_nulls is not None else None, preds=preds)
+
+    def results(self):
+        r = self.g.results()
+        keys = [None if r["key_nulls"][0][i] else self.names[int(k)] for i, k in enumerate(r["keys"][0])]
+        return {"count": r["count"], "keys": keys, "values": r["values"], "value_nulls": r["value_nulls"]}
+
+
 def query_mt(nthreads, aggs, key_types, value_type, keys, values=None, value_nulls=None,
              key_nulls=None, preds=(), out_types=None) -> GroupBy:
     """Threaded oracle run (CPU baseline): one table per thread, merged."""

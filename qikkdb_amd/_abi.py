@@ -195,6 +195,13 @@ PROTOTYPES = {
     "qk_batch_key_column": [_vp, _vp, _i32, _i32, _i32, _vp, _vp],
     "qk_batch_resample": [_vp],
     "qk_batch_estimate_rows": [_vp, _i32, _P(BatchQuery), _P(_i64)],
+    "qk_gbs_create": [_P(_vp), _i32, _i32, _P(_i32), _P(_i32), _i32, _i32, _u32],
+    "qk_gbs_destroy": [_vp],
+    "qk_gbs_reset": [_vp, _vp],
+    "qk_gbs_process_block": [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp],
+    "qk_gbs_process_block_where": [_vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _P(Predicate), _vp],
+    "qk_gbs_get_results": [_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_vp), _P(_i32), _vp],
+    "qk_gbs_group_count": [_vp, _P(_i64), _vp],
     "qk_mem_set_limit": [_sz],
     "qk_mem_set_limit_percent": [_i32],
     "qk_mem_get_limit": [_P(_sz)],

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 HIP_SOURCES = ["runtime.hip", "filter.hip", "reconstruct.hip", "groupby.hip", "twophase.hip", "batch.hip", "serve.hip",
-               "cache.hip"]
+               "cache.hip", "strings.hip"]
 LIB = os.path.join(PKG, "libqikkdb_amd.so")
 DATAGEN_LIB = os.path.join(PKG, "libqk_datagen.so")
 ARCH = "gfx950"
@@ -95,12 +95,14 @@ DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "groupby_driver.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "groupby_driver")
 DISPATCHER_SRC = os.path.join(ROOT, "tests", "cpp", "dispatcher_driver.cpp")
 DISPATCHER = os.path.join(ROOT, "tests", "cpp", "dispatcher_driver")
+STRING_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "string_driver.cpp")
+STRING_DRIVER = os.path.join(ROOT, "tests", "cpp", "string_driver")
 
 
 def build_driver(verbose: bool = False) -> str | None:
     """The C++ callers of the drop-in headers (include/qikkdb/) used by the parity tests."""
     out = None
-    for src, exe in ((DRIVER_SRC, DRIVER), (DISPATCHER_SRC, DISPATCHER)):
+    for src, exe in ((DRIVER_SRC, DRIVER), (DISPATCHER_SRC, DISPATCHER), (STRING_DRIVER_SRC, STRING_DRIVER)):
         if not os.path.exists(src):
             continue
         tmp = exe + ".tmp"

@@ -116,3 +116,12 @@ def test_dispatcher_shaped_caller(tmp_path):
             g.process(keys, v, _pack(vn), kn, preds=preds)
             want = as_map(g.results())
             assert got[(form, agg)] == want, f"{form} {agg} differs from the oracle"
+
+
+def test_string_caller_known_answers():
+    # the drop-in GPUGroupBy<AGG,O,std::string,V> from C++ on the reference's six string cases
+    r = subprocess.run([B.STRING_DRIVER], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    checks = {l.split()[1] for l in r.stdout.splitlines() if l.startswith("CHECK")}
+    assert checks == {"StringUnique", "StringSimple", "StringMultiBlockSimple", "StringMultiBlockMediumSum",
+                      "StringMultiBlockMediumMin", "StringMultiBlockMediumMax"}

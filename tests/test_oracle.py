@@ -295,3 +295,28 @@ def test_compare_groups_catches_planted_differences():
             c["values"][0][3] *= 1 + 3e-5
         with pytest.raises(AssertionError):
             O.compare_groups(c, a, rel_aggs=(0,), rel=1e-5)
+
+
+# ---- string keys ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", G["group_by_string"], ids=lambda c: c["source"].split()[1])
+def test_group_by_string_known_answers(case):
+    g = O.GroupByStrings(AGG_NAMES[case["agg"]], O.INT)
+    for keys, vals in zip(case["blocks"], case["values"]):
+        g.process(keys, np.array(vals, np.int32))
+    r = g.results()
+    got = {k.decode(): int(v) for k, v in zip(r["keys"], r["values"][0])}
+    assert got == case["expect"]
+
+
+def test_group_by_string_exact_bytes_nulls_and_where():
+    # byte strings that differ only in a suffix, a prefix or a trailing NUL byte stay apart; the
+    # empty string is a key of its own, distinct from the NULL key
+    keys = [b"ab", b"ab\x00", b"a", b"", b"ab", b"", b"b", b"ab\x00"]
+    vals = np.array([1, 2, 4, 8, 16, 32, 64, 128], np.int32)
+    knull = pack_bits(np.array([0, 0, 0, 0, 0, 0, 1, 0], bool))
+    g = O.GroupByStrings([O.AGG_SUM, O.AGG_COUNT], O.INT)
+    g.process(keys, vals, key_nulls=knull, preds=[O.pred(1, vals, 2, O.INT)])  # value >= 2
+    r = g.results()
+    got = {k: (int(s), int(c)) for k, s, c in zip(r["keys"], r["values"][0], r["values"][1])}
+    assert got == {b"ab\x00": (130, 2), b"a": (4, 1), b"": (40, 2), b"ab": (16, 1), None: (64, 1)}
