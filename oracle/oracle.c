@@ -22,6 +22,10 @@
  *   S8  MIN starts at max(), MAX at lowest(), SUM/AVG/COUNT at 0 (AggregationFunctions.cuh).
  * Floating SUM/AVG are accumulated in FP64 here: the reference's FP32 atomics have no fixed
  * order (AggregationFunctions.cuh:219-223), so the FP64 value is the reference point.
+ * NaN and signed zero, where the reference's result depends on row order, follow a chosen
+ * rule instead (N1): a FLOAT/DOUBLE key -0.0 groups with +0.0 and every NaN key falls in one
+ * group; float MIN/MAX follow IEEE 754 totalOrder, so a positive NaN is the MAX of its group
+ * and is skipped by MIN unless the group holds nothing else (see key_bits and total_lt).
  */
 #include "oracle.h"
 
@@ -196,7 +200,9 @@ struct or_gb {
 static int is_real(int t) { return t == OR_FLOAT || t == OR_DOUBLE; }
 
 /* Key bit pattern used for equality: integers sign-extended, floats by value with -0.0
- * folded into +0.0 so that `==` semantics hold for zeros. */
+ * folded into +0.0 so that `==` semantics hold for zeros, and every NaN folded into one
+ * quiet NaN so that NaN keys form one group (a deliberate, deterministic departure from
+ * GPUGroupBy.cuh:105, whose `!=` key test gives each NaN row a group of its own). */
 static uint64_t key_bits(int type, const void* col, int64_t i) {
     switch (type) {
     case OR_INT8: return (uint64_t)(int64_t)((const int8_t*)col)[i];
@@ -206,6 +212,7 @@ static uint64_t key_bits(int type, const void* col, int64_t i) {
         float f = ((const float*)col)[i];
         uint32_t b;
         if (f == 0.0f) f = 0.0f;
+        if (f != f) return 0x7fc00000u;
         memcpy(&b, &f, 4);
         return b;
     }
@@ -213,11 +220,32 @@ static uint64_t key_bits(int type, const void* col, int64_t i) {
         double d = ((const double*)col)[i];
         uint64_t b;
         if (d == 0.0) d = 0.0;
+        if (d != d) return 0x7ff8000000000000ull;
         memcpy(&b, &d, 8);
         return b;
     }
     }
 }
+
+/* Strict "a before b" in IEEE 754-2019 totalOrder (clause 5.10): -NaN < -inf < ... < -0.0 <
+ * +0.0 < ... < +inf < +NaN, NaNs of one sign ordered by payload.  Float MIN/MAX use this order:
+ * the reference's CAS loops (AggregationFunctions.cuh:44-80 and their max twins) restart from a
+ * NaN, so their result depends on row order and no parallel order can reproduce it. */
+static int64_t total_rank(double d) {
+    int64_t b;
+    memcpy(&b, &d, 8);
+    return b < 0 ? (int64_t)((uint64_t)b ^ 0x7fffffffffffffffull) : b;
+}
+static int total_lt(double a, double b) { return total_rank(a) < total_rank(b); }
+static double bits_double(uint64_t u) {
+    double d;
+    memcpy(&d, &u, 8);
+    return d;
+}
+/* The first and last values in totalOrder (a NaN of each sign with the largest payload): the
+ * MAX and MIN states of a group that has seen no value yet. */
+#define total_first bits_double(0xffffffffffffffffull)
+#define total_last bits_double(0x7fffffffffffffffull)
 
 /* FNV-1a over the key words (a different hash from the GPU table on purpose). */
 static uint64_t hash_key(int nk, const uint64_t* bits, const uint8_t* nulls) {
@@ -255,8 +283,8 @@ static int alloc_arrays(or_gb* g, int64_t cap) {
     for (int64_t s = 0; s < cap; s++) {
         g->min_i[s] = INT64_MAX;
         g->max_i[s] = INT64_MIN;
-        g->min_f[s] = INFINITY;
-        g->max_f[s] = -INFINITY;
+        g->min_f[s] = total_last;
+        g->max_f[s] = total_first;
     }
     return 0;
 }
@@ -393,8 +421,8 @@ void or_gb_process(or_gb* g, const void* const* keys, const uint64_t* const* key
             double v = vt == OR_FLOAT ? (double)((const float*)values)[i]
                                       : ((const double*)values)[i];
             g->sum_f[s] += v;
-            if (v < g->min_f[s]) g->min_f[s] = v;
-            if (v > g->max_f[s]) g->max_f[s] = v;
+            if (total_lt(v, g->min_f[s])) g->min_f[s] = v;
+            if (total_lt(g->max_f[s], v)) g->max_f[s] = v;
             break;
         }
         }
@@ -413,8 +441,8 @@ int or_gb_merge(or_gb* dst, const or_gb* src) {
         dst->sum_f[d] += src->sum_f[s];
         if (src->min_i[s] < dst->min_i[d]) dst->min_i[d] = src->min_i[s];
         if (src->max_i[s] > dst->max_i[d]) dst->max_i[d] = src->max_i[s];
-        if (src->min_f[s] < dst->min_f[d]) dst->min_f[d] = src->min_f[s];
-        if (src->max_f[s] > dst->max_f[d]) dst->max_f[d] = src->max_f[s];
+        if (total_lt(src->min_f[s], dst->min_f[d])) dst->min_f[d] = src->min_f[s];
+        if (total_lt(dst->max_f[d], src->max_f[s])) dst->max_f[d] = src->max_f[s];
     }
     return 0;
 }

@@ -88,14 +88,19 @@ __device__ __forceinline__ uint64_t next_in_sub(const GView& g, uint64_t s) {
 
 // ---- key normalisation -----------------------------------------------------------------
 
-// Bits of a FLOAT/DOUBLE key with -0.0 folded into +0.0 (so equal values group together).
+// Bits of a FLOAT/DOUBLE key with -0.0 folded into +0.0 (so equal values group together) and
+// every NaN (either sign, any payload) folded into the default quiet NaN, so that NaN keys form
+// one group.  The reference compares keys with `!=` (GPUGroupBy.cuh:105), under which a NaN
+// key matches no slot and each NaN row opens a group of its own.
 __device__ __forceinline__ uint64_t fkey32(float f) {
-    uint32_t b = __float_as_uint(f);
-    return (b & 0x7fffffffu) ? (uint64_t)b : 0ull;
+    const uint32_t b = __float_as_uint(f);
+    const uint32_t m = b & 0x7fffffffu;
+    return m > 0x7f800000u ? 0x7fc00000ull : (m ? (uint64_t)b : 0ull);
 }
 __device__ __forceinline__ uint64_t fkey64(double d) {
-    uint64_t b = (uint64_t)__double_as_longlong(d);
-    return (b & 0x7fffffffffffffffull) ? b : 0ull;
+    const uint64_t b = (uint64_t)__double_as_longlong(d);
+    const uint64_t m = b & 0x7fffffffffffffffull;
+    return m > 0x7ff0000000000000ull ? 0x7ff8000000000000ull : (m ? b : 0ull);
 }
 
 // Single key column widened to one 64-bit word.

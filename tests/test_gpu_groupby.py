@@ -589,3 +589,55 @@ def test_pair_key_equal_to_empty_marker(path):
     compare(gpu, cpu, len(aggs))
     m = as_map(gpu, 3)
     assert m[(0, int(np.iinfo(np.int32).min))] == int(np.count_nonzero((a == 0) & (b == np.iinfo(np.int32).min)))
+
+
+def _same_floats(x, y):
+    """Element-wise: both NaN with the same sign, or equal with the same sign (so -0.0 differs
+    from +0.0)."""
+    x, y = np.asarray(x, np.float64), np.asarray(y, np.float64)
+    nx, ny = np.isnan(x), np.isnan(y)
+    return (nx == ny) & (np.signbit(x) == np.signbit(y)) & (nx | (x == y))
+
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_nan_and_signed_zero_keys_and_values(dt, path):
+    """NaN and signed zeros as keys and values (the oracle's rule N1): -0.0 groups with +0.0,
+    all NaN keys form one group keyed by the positive quiet NaN, MIN/MAX follow IEEE totalOrder
+    (sign of NaNs and zeros included), SUM/AVG are NaN exactly where the oracle's are. Hot keys
+    from a small pool (aggregated in LDS on the two-phase path) mixed with 100K cold ones."""
+    rng = np.random.default_rng(31 if dt == np.float32 else 32)
+    n = 1 << 20
+    qnan_payload = np.array([0x7FC12345], np.uint32).view(np.float32)[0]
+    key_pool = np.array([0.0, -0.0, 1.5, np.nan, -np.nan, qnan_payload, np.inf, -np.inf], dt)
+    val_pool = np.array([1.0, -1.0, 0.0, -0.0, np.nan, -np.nan, np.inf, -np.inf, 2.5], dt)
+    hot = rng.random(n) < 0.6
+    keys = np.where(hot, key_pool[rng.integers(0, key_pool.size, n)],
+                    (rng.integers(1, 100_000, n) * 0.25).astype(dt)).astype(dt)
+    vals = np.where(rng.random(n) < 0.3, val_pool[rng.integers(0, val_pool.size, n)],
+                    rng.standard_normal(n).astype(dt)).astype(dt)
+    vn = rng.random(n) < 0.05
+    aggs = [qk.AGG_MIN, qk.AGG_MAX, qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT]
+    gpu = gpu_group_by(aggs, [keys], vals, None, vn, block=1 << 19, capacity=200_000, path=path)
+    cpu = cpu_group_by(aggs, [keys], vals, None, vn)
+    assert gpu["count"] == cpu["count"]
+    a, b = O.sort_groups(gpu), O.sort_groups(cpu)
+    assert np.array_equal(a["keys"][0], b["keys"][0])
+    nan_key = np.flatnonzero(np.isnan(np.asarray(gpu["keys"][0][:gpu["count"]], np.float64)))
+    assert nan_key.size == 1, "one NaN group"
+    assert not np.signbit(np.float64(gpu["keys"][0][nan_key[0]]))
+    zero = np.asarray(gpu["keys"][0][:gpu["count"]], np.float64) == 0
+    assert zero.sum() == 1 and not np.signbit(np.asarray(gpu["keys"][0][:gpu["count"]], np.float64)[zero]).any()
+    for i in range(len(aggs)):
+        assert np.array_equal(a["value_nulls"][i], b["value_nulls"][i])
+    live = ~b["value_nulls"][0]
+    for i in (0, 1):
+        ok = _same_floats(a["values"][i][live], b["values"][i][live])
+        assert ok.all(), f"aggregate {i}: {np.count_nonzero(~ok)} groups differ"
+    for i in (2, 3):
+        x, y = (np.asarray(v[live], np.float64) for v in (a["values"][i], b["values"][i]))
+        assert np.array_equal(np.isnan(x), np.isnan(y))
+        fin = ~np.isnan(y) & np.isfinite(y)
+        assert np.allclose(x[fin], y[fin], rtol=FP_REL, atol=1e-3)
+        assert np.array_equal(x[np.isinf(y)], y[np.isinf(y)])
+    assert np.array_equal(a["values"][4], b["values"][4])

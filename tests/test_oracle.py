@@ -2,6 +2,8 @@
 brute force on tiny inputs.  Each check is chosen so that a plausible mistake in the oracle
 (wrong operand conversion, dropped NULL rule, wrong AVG rounding, lost int32 wrap, wrong
 index offset) fails it."""
+import math
+
 import numpy as np
 import pytest
 
@@ -320,3 +322,84 @@ def test_group_by_string_exact_bytes_nulls_and_where():
     r = g.results()
     got = {k: (int(s), int(c)) for k, s, c in zip(r["keys"], r["values"][0], r["values"][1])}
     assert got == {b"ab\x00": (130, 2), b"a": (4, 1), b"": (40, 2), b"ab": (16, 1), None: (64, 1)}
+
+
+def _sign_class(x: float):
+    """(is NaN, sign bit, value) of a float: equal for two floats that are the same value,
+    telling -0.0 from +0.0 and NaNs apart by sign only."""
+    x = float(x)
+    return (math.isnan(x), math.copysign(1.0, x) < 0, None if math.isnan(x) else x)
+
+
+def _total_min_max(vals):
+    """MIN and MAX of a group's float values under IEEE totalOrder, stated through numpy's
+    nanmin/nanmax and the signs of the NaNs and zeros present: a positive NaN is the largest
+    value, a negative NaN the smallest, and -0.0 lies below +0.0."""
+    v = np.asarray(vals, np.float64)
+    nan, neg = np.isnan(v), np.signbit(v)
+    num = v[~nan]
+    if (nan & neg).any():
+        mn = -math.nan
+    elif num.size:
+        mn = float(np.nanmin(num))
+        if mn == 0.0:
+            mn = -0.0 if (neg & (v == 0)).any() else 0.0
+    else:
+        mn = math.nan
+    if (nan & ~neg).any():
+        mx = math.nan
+    elif num.size:
+        mx = float(np.nanmax(num))
+        if mx == 0.0:
+            mx = 0.0 if (~neg & (v == 0)).any() else -0.0
+    else:
+        mx = -math.nan
+    return mn, mx
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_nan_and_signed_zero_rules(dt, seed):
+    """N1: -0.0 and +0.0 keys form one group and every NaN key (any sign or payload) one group,
+    grouped here by Python's own float equality with NaN mapped to one marker; MIN/MAX follow
+    totalOrder (closed form above); SUM is NaN exactly when numpy's FP64 sum is."""
+    rng = np.random.default_rng(seed)
+    qnan_payload = np.array([0x7FC12345], np.uint32).view(np.float32)[0]
+    key_pool = np.array([0.0, -0.0, 1.5, -1.5, np.nan, -np.nan, qnan_payload, np.inf, -np.inf], dt)
+    val_pool = np.array([1.0, -1.0, 0.0, -0.0, np.nan, -np.nan, np.inf, -np.inf, 2.5, -7.25], dt)
+    n = 400
+    keys = key_pool[rng.integers(0, key_pool.size, n)]
+    vals = val_pool[rng.integers(0, val_pool.size, n)]
+    vnull = rng.random(n) < 0.1
+    ft = O.FLOAT if dt == np.float32 else O.DOUBLE
+    aggs = [AGG_NAMES[a] for a in ("MIN", "MAX", "SUM", "COUNT")]
+    g = O.GroupBy(aggs, ft, ft)
+    g.process([keys], vals, pack_bits(vnull))
+    r = g.results()
+    want = {}
+    for k, v, isnull in zip(keys.tolist(), vals.tolist(), vnull.tolist()):
+        e = want.setdefault("nan" if math.isnan(k) else k, [])
+        if not isnull:
+            e.append(v)
+    assert r["count"] == len(want)
+    seen = set()
+    for i in range(r["count"]):
+        k = float(r["keys"][0][i])
+        gk = "nan" if math.isnan(k) else k
+        assert gk not in seen
+        seen.add(gk)
+        assert not math.copysign(1.0, k) < 0 or k != 0.0, "a zero key is +0.0"
+        if math.isnan(k):
+            assert not math.copysign(1.0, k) < 0, "the NaN key is the positive quiet NaN"
+        gv = want[gk]
+        assert int(r["values"][3][i]) == len(gv)
+        if not gv:
+            assert r["value_nulls"][0][i] and r["value_nulls"][1][i]
+            continue
+        mn, mx = _total_min_max(gv)
+        assert _sign_class(r["values"][0][i]) == _sign_class(mn), (gk, gv)
+        assert _sign_class(r["values"][1][i]) == _sign_class(mx), (gk, gv)
+        s = float(np.sum(np.asarray(gv, np.float64)))
+        assert math.isnan(float(r["values"][2][i])) == math.isnan(s)
+        if not math.isnan(s):
+            assert float(r["values"][2][i]) == pytest.approx(s, rel=1e-6)
