@@ -109,6 +109,40 @@ __device__ __host__ __forceinline__ uint64_t hash64(uint64_t k) {
     return k ^ (k >> 32);
 }
 
+// Global-memory views of pointers that kernels load from memory (table views, argument
+// structs).  The compiler cannot tell where such a pointer points and accesses it with flat
+// instructions, which count against both the vector-memory and the LDS wait counters: a wait
+// for any earlier load then drains every outstanding load.  Device allocations are global
+// memory, so accesses through these views compile to global instructions instead.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_g(T* p) {
+    return (gptr<T>)p;
+}
+// Relaxed, device-scope atomics on global memory (what atomicCAS / atomicAdd / atomicMin /
+// atomicMax do on a generic pointer); each returns the old value.
+__device__ __forceinline__ unsigned long long g_atomic_cas(unsigned long long* p, unsigned long long cmp,
+                                                           unsigned long long val) {
+    __hip_atomic_compare_exchange_strong(to_g(p), &cmp, val, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return cmp;
+}
+template <typename T>
+__device__ __forceinline__ T g_atomic_add(T* p, T v) {
+    return __hip_atomic_fetch_add(to_g(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long g_atomic_min(unsigned long long* p, unsigned long long v) {
+    return __hip_atomic_fetch_min(to_g(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long g_atomic_max(unsigned long long* p, unsigned long long v) {
+    return __hip_atomic_fetch_max(to_g(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T g_load_volatile(const T* p) {
+    return *(const volatile __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

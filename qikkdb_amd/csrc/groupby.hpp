@@ -203,7 +203,7 @@ __device__ __forceinline__ Part row_part(const VT& v, bool vnull) {
 // ---- global table: probing with atomics (any number of writers) --------------------------
 
 __device__ __forceinline__ void g_set_error(const GView& g, int32_t code) {
-    __hip_atomic_store(g.err, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(to_g(g.err), code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // A partial whose key found no free slot in its sub-table: the table reports itself full and
@@ -211,15 +211,15 @@ __device__ __forceinline__ void g_set_error(const GView& g, int32_t code) {
 // dropped partials are exactly those whose key is absent from the table afterwards (every
 // partial of a key the table holds was aggregated); a growing table re-aggregates those rows.
 __device__ __forceinline__ void g_set_lost(const GView& g) {
-    __hip_atomic_store(g.lost, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(to_g(g.lost), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     g_set_error(g, QK_HASH_TABLE_FULL);
 }
 
 // Side slot claim (0 -> 1); returns true when this call claimed it.
 __device__ __forceinline__ bool g_claim_side(const GView& g, uint64_t s) {
     uint64_t* kp = g.keys + s * (uint64_t)g.kw;
-    if (*(volatile uint64_t*)kp != 0ull) return false;
-    return atomicCAS((unsigned long long*)kp, 0ull, 1ull) == 0ull;
+    if (g_load_volatile(kp) != 0ull) return false;
+    return g_atomic_cas((unsigned long long*)kp, 0ull, 1ull) == 0ull;
 }
 
 // Slot of a single-word key (inserting it when absent); -1 when its sub-table is full.
@@ -238,11 +238,11 @@ __device__ __forceinline__ int64_t g_slot1(const GView& g, uint64_t key, bool kn
     uint64_t s = home_of(g, h);
     for (uint64_t p = 0; p <= g.sub_mask; p++) {
         uint64_t* kp = g.keys + s;
-        uint64_t cur = *(volatile uint64_t*)kp;  // a stale kEmpty only costs a CAS
+        uint64_t cur = g_load_volatile(kp);  // a stale kEmpty only costs a CAS
         if (cur == key) return (int64_t)s;
         if (cur == kEmpty) {
-            unsigned long long old = atomicCAS((unsigned long long*)kp, (unsigned long long)kEmpty,
-                                               (unsigned long long)key);
+            unsigned long long old = g_atomic_cas((unsigned long long*)kp, (unsigned long long)kEmpty,
+                                                  (unsigned long long)key);
             if (old == kEmpty) {
                 *is_new = true;
                 return (int64_t)s;
@@ -260,8 +260,8 @@ __device__ __forceinline__ int64_t g_slot1_claim(const GView& g, uint64_t key, u
     *is_new = false;
     uint64_t s = home_of(g, h);
     for (uint64_t p = 0; p <= g.sub_mask; p++) {
-        const unsigned long long old = atomicCAS((unsigned long long*)(g.keys + s), (unsigned long long)kEmpty,
-                                                 (unsigned long long)key);
+        const unsigned long long old = g_atomic_cas((unsigned long long*)(g.keys + s), (unsigned long long)kEmpty,
+                                                    (unsigned long long)key);
         if (old == kEmpty) {
             *is_new = true;
             return (int64_t)s;
@@ -342,20 +342,20 @@ __device__ __forceinline__ bool g_hasN(const GView& g, const uint64_t (&k)[KW], 
 
 // Atomic merge of a partial into slot s (concurrent writers allowed).
 __device__ __forceinline__ void g_accum_atomic(const GView& g, int64_t s, const Part& p, uint32_t fields) {
-    if (p.cnt) atomicAdd(g.cnt + s, (unsigned long long)p.cnt);
-    if ((fields & F_CNTALL) && p.cntall) atomicAdd(g.cntall + s, (unsigned long long)p.cntall);
+    if (p.cnt) g_atomic_add((unsigned long long*)(g.cnt + s), (unsigned long long)p.cnt);
+    if ((fields & F_CNTALL) && p.cntall) g_atomic_add((unsigned long long*)(g.cntall + s), (unsigned long long)p.cntall);
     if (p.cnt) {
         if (fields & F_SUM) {
             if (g.real_sum)
-                atomicAdd((double*)(g.sum + s), __longlong_as_double((long long)p.sum));
+                g_atomic_add((double*)(g.sum + s), __longlong_as_double((long long)p.sum));
             else
-                atomicAdd(g.sum + s, (unsigned long long)p.sum);
+                g_atomic_add((unsigned long long*)(g.sum + s), (unsigned long long)p.sum);
         }
         // the read first skips atomics that cannot change the value
-        if ((fields & F_MIN) && p.mn < *(volatile unsigned long long*)(g.mn + s))
-            atomicMin(g.mn + s, (unsigned long long)p.mn);
-        if ((fields & F_MAX) && p.mx > *(volatile unsigned long long*)(g.mx + s))
-            atomicMax(g.mx + s, (unsigned long long)p.mx);
+        if ((fields & F_MIN) && p.mn < g_load_volatile(g.mn + s))
+            g_atomic_min((unsigned long long*)(g.mn + s), (unsigned long long)p.mn);
+        if ((fields & F_MAX) && p.mx > g_load_volatile(g.mx + s))
+            g_atomic_max((unsigned long long*)(g.mx + s), (unsigned long long)p.mx);
     }
 }
 
@@ -389,7 +389,7 @@ __device__ __forceinline__ void g_store_new(const GView& g, int64_t s, const Par
 // Count `n` new keys of one workgroup; reports a full table against the logical capacity.
 __device__ __forceinline__ void g_count_new(const GView& g, uint64_t n) {
     if (!n) return;
-    unsigned long long c = atomicAdd(g.nkeys, (unsigned long long)n) + n;
+    unsigned long long c = g_atomic_add(g.nkeys, (unsigned long long)n) + n;
     if (c > g.logical_cap) g_set_error(g, QK_HASH_TABLE_FULL);
 }
 

@@ -417,17 +417,21 @@ __host__ __device__ inline size_t p1_lds_bytes(int32_t H, int32_t P, int32_t vby
     return b;
 }
 
-// Slot of key k in the heavy table, or -1 (one bucket of kBucket keys, two 128-bit reads).
+// Slot of key k in the heavy table, or -1 (one bucket of kBucket keys, read 128 bits at a time).
 __device__ __forceinline__ int heavy_find(const uint64_t* lkey, uint32_t nb, uint64_t k) {
+    static_assert(kBucket == 2 || kBucket == 4, "bucket width");
     const uint32_t b = bucket_of(k, nb);
     const ulonglong2* p = reinterpret_cast<const ulonglong2*>(lkey + (size_t)b * kBucket);
-    const ulonglong2 a0 = p[0], a1 = p[1];
     const int base = (int)(b * kBucket);
+    const ulonglong2 a0 = p[0];
     int s = -1;
     s = a0.x == k ? base : s;
     s = a0.y == k ? base + 1 : s;
-    s = a1.x == k ? base + 2 : s;
-    s = a1.y == k ? base + 3 : s;
+    if constexpr (kBucket == 4) {
+        const ulonglong2 a1 = p[1];
+        s = a1.x == k ? base + 2 : s;
+        s = a1.y == k ? base + 3 : s;
+    }
     return s;
 }
 
@@ -449,6 +453,190 @@ constexpr bool kTypedKey = !std::is_same<KT, void>::value && !kPairKey<KT>;
 // them from `fields`.
 constexpr int kFieldsAny = 0, kFieldsSum = 1;
 
+// The phase-1 LDS of one workgroup, carved as p1_lds_bytes lays it out.
+template <typename VV>
+struct P1Lds {
+    uint64_t* lkey;
+    uint64_t* lsum;
+    uint64_t* lmin;
+    uint64_t* lmax;
+    uint64_t* qkey;   // this wavefront's cold-row ring: keys
+    VV* qval;         // and values
+    uint32_t* lcnt;   // rows with a non-NULL value, per slot
+    uint32_t* lnul;   // rows with a NULL value, per slot
+    uint32_t* cursor; // per partition: records in this workgroup's region
+    uint32_t* nnew;   // keys the atomic path added to the global table
+};
+
+// Lays out the LDS, copies the heavy table's image in, clears the states and loads the
+// region cursors (continuing earlier blocks' regions when accumulating).  Ends with a barrier.
+template <typename VT, typename VV>
+__device__ __forceinline__ P1Lds<VV> p1_begin(uint64_t* lds, const P1Args& a, bool f_sum, bool f_min,
+                                              bool f_max) {
+    constexpr bool HAS_V = VTraits<VT>::has;
+    const int H = a.H, T = H + 2, bd = blockDim.x, tid = threadIdx.x, wave = tid >> 6;
+    P1Lds<VV> L;
+    L.lkey = lds;
+    uint64_t* p = L.lkey + H;
+    L.lsum = f_sum ? p : nullptr;
+    p += f_sum ? T : 0;
+    L.lmin = f_min ? p : nullptr;
+    p += f_min ? T : 0;
+    L.lmax = f_max ? p : nullptr;
+    p += f_max ? T : 0;
+    // the cold-row rings of the wavefronts: keys [waves][kQueue], then values
+    L.qkey = p + (size_t)wave * kQueue;
+    p += (size_t)(kBlock / 64) * kQueue;
+    L.qval = reinterpret_cast<VV*>(p) + (size_t)wave * kQueue;
+    p = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(p) + (size_t)(kBlock / 64) * kQueue * (HAS_V ? sizeof(VT) : 0));
+    L.lcnt = (uint32_t*)p;
+    L.lnul = L.lcnt + T;
+    L.cursor = L.lnul + T;
+    L.nnew = L.cursor + a.P;
+    // the heavy table's keys from the image (two keys per 16-byte load), states cleared
+    {
+        const ulonglong2* img = reinterpret_cast<const ulonglong2*>(a.image);
+        ulonglong2* dst = reinterpret_cast<ulonglong2*>(L.lkey);
+        for (int i = tid; i < H / 2; i += bd) dst[i] = img[i];
+    }
+    for (int s = tid; s < T; s += bd) {
+        L.lcnt[s] = 0;
+        L.lnul[s] = 0;
+        if (L.lsum) L.lsum[s] = 0;
+        if (L.lmin) L.lmin[s] = ~0ull;
+        if (L.lmax) L.lmax[s] = 0;
+    }
+    const int64_t region0 = (int64_t)blockIdx.x * a.P;
+    // regions continue where the previous block left them until phase 2 consumes them
+    for (int q = tid; q < a.P; q += bd) L.cursor[q] = a.accumulate ? a.r_count[region0 + q] : 0u;
+    if (tid == 0) L.nnew[0] = 0;
+    __syncthreads();
+    return L;
+}
+
+// Writes the workgroup's heavy partials (indexed by heavy key; zeros for a key the image could
+// not place), added to what earlier blocks left when phase 2 has not consumed it yet, the
+// region fill counts and the count of keys the atomic path added.  Starts with a barrier.
+template <bool REAL, typename VV>
+__device__ __forceinline__ void p1_end(const P1Lds<VV>& L, const P1Args& a, const GView* gp) {
+    __syncthreads();
+    const int H = a.H, bd = blockDim.x, tid = threadIdx.x;
+    const int32_t K = min(a.meta[0], a.kmax);
+    const int64_t pbase = (int64_t)blockIdx.x * (a.kmax + 2);
+    for (int k = tid; k < K + 2; k += bd) {
+        const int s = k < K ? a.slot[k] : H + (k - K);
+        const int64_t at = pbase + (k < K ? k : a.kmax + (k - K));
+        uint32_t cnt = s >= 0 ? L.lcnt[s] : 0u, rows = s >= 0 ? L.lcnt[s] + L.lnul[s] : 0u;
+        uint64_t sum = (L.lsum && s >= 0) ? L.lsum[s] : 0ull;
+        uint64_t mn = (L.lmin && s >= 0) ? L.lmin[s] : ~0ull, mx = (L.lmax && s >= 0) ? L.lmax[s] : 0ull;
+        if (a.accumulate) {
+            rows += a.p_rows[at];
+            cnt += a.p_cnt[at];
+            if (L.lsum) {
+                if (REAL)
+                    sum = (uint64_t)__double_as_longlong(__longlong_as_double((long long)sum) +
+                                                         __longlong_as_double((long long)a.p_sum[at]));
+                else
+                    sum += a.p_sum[at];
+            }
+            if (L.lmin) mn = min(mn, (uint64_t)a.p_min[at]);
+            if (L.lmax) mx = max(mx, (uint64_t)a.p_max[at]);
+        }
+        a.p_rows[at] = rows;
+        a.p_cnt[at] = cnt;
+        if (L.lsum) a.p_sum[at] = sum;
+        if (L.lmin) a.p_min[at] = mn;
+        if (L.lmax) a.p_max[at] = mx;
+    }
+    const int64_t region0 = (int64_t)blockIdx.x * a.P;
+    for (int q = tid; q < a.P; q += bd) a.r_count[region0 + q] = min(L.cursor[q], (uint32_t)a.cap);
+    if (tid == 0) g_count_new(*gp, L.nnew[0]);
+}
+
+// One heavy row into its LDS slot l.
+template <typename VT, typename VV>
+__device__ __forceinline__ void p1_heavy_add(const P1Lds<VV>& L, int l, const VV& v, bool vn) {
+    constexpr bool HAS_V = VTraits<VT>::has;
+    constexpr bool REAL = VTraits<VT>::real;
+    Part pr;
+    if constexpr (HAS_V) pr = row_part<VT>(v, vn); else pr = row_part<NoValue>(NoValue{}, vn);
+    if (pr.cnt) {
+        atomicAdd(&L.lcnt[l], 1u);
+        if (L.lsum) {
+            if (REAL)
+                atomicAdd((double*)&L.lsum[l], __longlong_as_double((long long)pr.sum));
+            else
+                atomicAdd((unsigned long long*)&L.lsum[l], (unsigned long long)pr.sum);
+        }
+        if (L.lmin) atomicMin((unsigned long long*)&L.lmin[l], (unsigned long long)pr.mn);
+        if (L.lmax) atomicMax((unsigned long long*)&L.lmax[l], (unsigned long long)pr.mx);
+    } else {
+        atomicAdd(&L.lnul[l], 1u);
+    }
+}
+
+// A row that cannot go to a region (NULL value, or its region is full): straight into the
+// global table with atomics.
+template <typename VT, typename VV>
+__device__ __forceinline__ void p1_atomic_row(const GView* gp, uint32_t* nnew, uint32_t fields, uint64_t kk,
+                                              uint64_t h, const VV& v, bool vn) {
+    Part pr;
+    if constexpr (VTraits<VT>::has) pr = row_part<VT>(v, vn); else pr = row_part<NoValue>(NoValue{}, vn);
+    const GView& g = *gp;
+    bool is_new = false;
+    const int64_t gs = g_slot1(g, kk, false, h, &is_new);
+    if (is_new) atomicAdd(nnew, 1u);
+    if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_lost(g);
+}
+
+// This wavefront's cold-row ring.  Head and count are the same on every lane (they only
+// change by ballot counts), so the ring's control flow is uniform.
+template <typename VT, typename VV>
+struct P1Ring {
+    uint32_t head = 0, cnt = 0;
+
+    // Partition and write the ring's first `n` (<= 64) rows, one per lane.
+    __device__ __forceinline__ void drain(const P1Lds<VV>& L, const P1Args& a, const GView* gp, uint32_t fields,
+                                          uint32_t n) {
+        const int lane = threadIdx.x & 63;
+        const bool live = (uint32_t)lane < n;
+        const uint32_t at_q = (head + (uint32_t)lane) & (kQueue - 1);
+        const uint64_t kk = L.qkey[at_q];
+        VV v = 0;
+        if constexpr (VTraits<VT>::has) v = L.qval[at_q];
+        const uint64_t h = hash64(kk);
+        // part_shift is 54..64; the split shift gives 0 for 64 without a select
+        const uint32_t q = (uint32_t)((h >> 1) >> (a.part_shift - 1));
+        uint32_t c = ~0u;
+        if (live) c = atomicAdd(&L.cursor[q], 1u);
+        if (live && c < (uint32_t)a.cap) {
+            put_rec<VT>(a.r_rec, ((int64_t)blockIdx.x * a.P + q) * (int64_t)a.cap + c, kk, v);
+        } else if (live) {
+            p1_atomic_row<VT>(gp, L.nnew, fields, kk, h, v, false);
+        }
+        head = (head + n) & (kQueue - 1);
+    }
+
+    // Queue the lanes' cold row (cold: this lane has one), draining 64 when that many wait.
+    __device__ __forceinline__ void push(const P1Lds<VV>& L, const P1Args& a, const GView* gp, uint32_t fields,
+                                         bool cold, uint64_t key, const VV& v) {
+        const uint64_t m = __ballot(cold);
+        if (!m) return;
+        const int lane = threadIdx.x & 63;
+        if (cold) {
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint32_t at_q = (head + cnt + (uint32_t)__popcll(m & below)) & (kQueue - 1);
+            L.qkey[at_q] = key;
+            if constexpr (VTraits<VT>::has) L.qval[at_q] = v;
+        }
+        cnt += (uint32_t)__popcll(m);
+        if (cnt >= 64) {
+            drain(L, a, gp, fields, 64);
+            cnt -= 64;
+        }
+    }
+};
+
 template <typename VT, int MODE, typename KT, int FP>
 __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec ks, KeyCols kc,
                                                       const VT* __restrict__ vals,
@@ -458,51 +646,14 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
                                                       P1Args a) {
     extern __shared__ __align__(16) uint64_t lds[];
     constexpr bool HAS_V = VTraits<VT>::has;
-    constexpr bool REAL = VTraits<VT>::real;
     using VV = typename std::conditional<HAS_V, VT, int>::type;
     const bool f_sum = FP == kFieldsSum ? true : (fields & F_SUM) != 0;
     const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
     const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
-    const int H = a.H, T = H + 2, bd = blockDim.x, tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int H = a.H, bd = blockDim.x, tid = threadIdx.x;
+    const int lane = tid & 63;
     const uint32_t nb = (uint32_t)(H / kBucket);
-    uint64_t* lkey = lds;
-    uint64_t* p = lkey + H;
-    uint64_t* lsum = f_sum ? p : nullptr;
-    p += f_sum ? T : 0;
-    uint64_t* lmin = f_min ? p : nullptr;
-    p += f_min ? T : 0;
-    uint64_t* lmax = f_max ? p : nullptr;
-    p += f_max ? T : 0;
-    // the cold-row rings of the wavefronts: keys [waves][kQueue], then values
-    uint64_t* qkey = p + (size_t)wave * kQueue;
-    p += (size_t)(kBlock / 64) * kQueue;
-    VV* qval = reinterpret_cast<VV*>(p) + (size_t)wave * kQueue;
-    p = reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(p) + (size_t)(kBlock / 64) * kQueue * (HAS_V ? sizeof(VT) : 0));
-    uint32_t* lcnt = (uint32_t*)p;   // rows with a non-NULL value
-    uint32_t* lnul = lcnt + T;       // rows with a NULL value
-    uint32_t* cursor = lnul + T;
-    uint32_t* nnew = cursor + a.P;
-
-    // the heavy table's keys from the image (two keys per 16-byte load), states cleared
-    {
-        const ulonglong2* img = reinterpret_cast<const ulonglong2*>(a.image);
-        ulonglong2* dst = reinterpret_cast<ulonglong2*>(lkey);
-        for (int i = tid; i < H / 2; i += bd) dst[i] = img[i];
-    }
-    for (int s = tid; s < T; s += bd) {
-        lcnt[s] = 0;
-        lnul[s] = 0;
-        if (lsum) lsum[s] = 0;
-        if (lmin) lmin[s] = ~0ull;
-        if (lmax) lmax[s] = 0;
-    }
-    const int64_t region0 = (int64_t)blockIdx.x * a.P;
-    // regions continue where the previous block left them until phase 2 consumes them
-    for (int q = tid; q < a.P; q += bd) cursor[q] = a.accumulate ? a.r_count[region0 + q] : 0u;
-    if (tid == 0) nnew[0] = 0;
-    __syncthreads();
-    const int32_t K = min(a.meta[0], a.kmax);
+    const P1Lds<VV> L = p1_begin<VT, VV>(lds, a, f_sum, f_min, f_max);
 
     const int64_t tile_rows = (int64_t)bd * kP1Steps;
     const int64_t ntiles = (n + tile_rows - 1) / tile_rows;
@@ -584,40 +735,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
             o.knull |= (uint32_t)kn << k;
         }
     };
-    // A row that cannot go to a region (NULL value, or its region is full): straight into the
-    // global table with atomics.
-    auto atomic_row = [&](uint64_t kk, uint64_t h, const VV& v, bool vn) {
-        Part pr;
-        if constexpr (HAS_V) pr = row_part<VT>(v, vn); else pr = row_part<NoValue>(NoValue{}, vn);
-        const GView& g = *gp;
-        bool is_new = false;
-        const int64_t gs = g_slot1(g, kk, false, h, &is_new);
-        if (is_new) atomicAdd(nnew, 1u);
-        if (gs >= 0) g_accum_atomic(g, gs, pr, fields); else g_set_lost(g);
-    };
-    // This wavefront's ring: head and count are the same on every lane (they only change by
-    // ballot counts), so the ring's control flow is uniform.
-    uint32_t qhead = 0, qcnt = 0;
-    const uint64_t below = (1ull << lane) - 1ull;
-    // Partition and write the ring's first `cnt` (<= 64) rows, one per lane.
-    auto drain = [&](uint32_t cnt) {
-        const bool live = (uint32_t)lane < cnt;
-        const uint32_t at_q = (qhead + (uint32_t)lane) & (kQueue - 1);
-        const uint64_t kk = qkey[at_q];
-        VV v = 0;
-        if constexpr (HAS_V) v = qval[at_q];
-        const uint64_t h = hash64(kk);
-        // part_shift is 54..64; the split shift gives 0 for 64 without a select
-        const uint32_t q = (uint32_t)((h >> 1) >> (a.part_shift - 1));
-        uint32_t c = ~0u;
-        if (live) c = atomicAdd(&cursor[q], 1u);
-        if (live && c < (uint32_t)a.cap) {
-            put_rec<VT>(a.r_rec, (region0 + q) * (int64_t)a.cap + c, kk, v);
-        } else if (live) {
-            atomic_row(kk, h, v, false);
-        }
-        qhead = (qhead + cnt) & (kQueue - 1);
-    };
+    P1Ring<VT, VV> ring;
     auto aggregate_rows = [&](const Rows& o) {
         // every step's bucket is read first (a NULL key reads key 0's; the NULL key and the
         // key equal to the empty marker then take their side slots by selection)
@@ -625,51 +743,22 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
             const uint64_t kk = o.key[k];
-            int l = heavy_find(lkey, nb, kk);
+            int l = heavy_find(L.lkey, nb, kk);
             l = kk == kEmpty ? H + 1 : l;
             ls[k] = ((o.knull >> k) & 1) ? H : l;
         }
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
             const bool pass = (o.pass >> k) & 1, vn = (o.vnull >> k) & 1;
-            if (pass && ls[k] >= 0) {
-                Part pr;
-                if constexpr (HAS_V) pr = row_part<VT>(o.val[k], vn); else pr = row_part<NoValue>(NoValue{}, vn);
-                const int l = ls[k];
-                if (pr.cnt) {
-                    atomicAdd(&lcnt[l], 1u);
-                    if (lsum) {
-                        if (REAL)
-                            atomicAdd((double*)&lsum[l], __longlong_as_double((long long)pr.sum));
-                        else
-                            atomicAdd((unsigned long long*)&lsum[l], (unsigned long long)pr.sum);
-                    }
-                    if (lmin) atomicMin((unsigned long long*)&lmin[l], (unsigned long long)pr.mn);
-                    if (lmax) atomicMax((unsigned long long*)&lmax[l], (unsigned long long)pr.mx);
-                } else {
-                    atomicAdd(&lnul[l], 1u);
-                }
-            }
+            if (pass && ls[k] >= 0) p1_heavy_add<VT>(L, ls[k], o.val[k], vn);
         }
         // cold rows with a value go to the ring; a NULL value takes the atomic path
 #pragma unroll
         for (int k = 0; k < kP1Steps; k++) {
             const bool pass = (o.pass >> k) & 1, vn = (o.vnull >> k) & 1;
             const bool cold = pass & (ls[k] < 0);
-            if (cold && vn) atomic_row(o.key[k], hash64(o.key[k]), o.val[k], true);
-            const uint64_t m = __ballot(cold && !vn);
-            if (m) {
-                if (cold && !vn) {
-                    const uint32_t at_q = (qhead + qcnt + (uint32_t)__popcll(m & below)) & (kQueue - 1);
-                    qkey[at_q] = o.key[k];
-                    if constexpr (HAS_V) qval[at_q] = o.val[k];
-                }
-                qcnt += (uint32_t)__popcll(m);
-                if (qcnt >= 64) {
-                    drain(64);
-                    qcnt -= 64;
-                }
-            }
+            if (cold && vn) p1_atomic_row<VT>(gp, L.nnew, fields, o.key[k], hash64(o.key[k]), o.val[k], true);
+            ring.push(L, a, gp, fields, cold && !vn, o.key[k], o.val[k]);
         }
     };
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -679,39 +768,222 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase1(const GView* gp, KeySpec k
         filter_tile(tile, t, o);
         aggregate_rows(o);
     }
-    if (qcnt) drain(qcnt);
+    if (ring.cnt) ring.drain(L, a, gp, fields, ring.cnt);
+    p1_end<VTraits<VT>::real>(L, a, gp);
+}
 
-    __syncthreads();
-    const int64_t pbase = (int64_t)blockIdx.x * (a.kmax + 2);
-    // heavy partials, indexed by heavy key (zeros for a key the image could not place),
-    // added to what earlier blocks left when phase 2 has not consumed it yet
-    for (int k = tid; k < K + 2; k += bd) {
-        const int s = k < K ? a.slot[k] : H + (k - K);
-        const int64_t at = pbase + (k < K ? k : a.kmax + (k - K));
-        uint32_t cnt = s >= 0 ? lcnt[s] : 0u, rows = s >= 0 ? lcnt[s] + lnul[s] : 0u;
-        uint64_t sum = (lsum && s >= 0) ? lsum[s] : 0ull;
-        uint64_t mn = (lmin && s >= 0) ? lmin[s] : ~0ull, mx = (lmax && s >= 0) ? lmax[s] : 0ull;
-        if (a.accumulate) {
-            rows += a.p_rows[at];
-            cnt += a.p_cnt[at];
-            if (lsum) {
-                if (REAL)
-                    sum = (uint64_t)__double_as_longlong(__longlong_as_double((long long)sum) +
-                                                         __longlong_as_double((long long)a.p_sum[at]));
-                else
-                    sum += a.p_sum[at];
-            }
-            if (lmin) mn = min(mn, (uint64_t)a.p_min[at]);
-            if (lmax) mx = max(mx, (uint64_t)a.p_max[at]);
-        }
-        a.p_rows[at] = rows;
-        a.p_cnt[at] = cnt;
-        if (lsum) a.p_sum[at] = sum;
-        if (lmin) a.p_min[at] = mn;
-        if (lmax) a.p_max[at] = mx;
+// ---- phase 1, streaming form ----------------------------------------------------------------
+//
+// The same work as k_tp_phase1 for one typed key column (int32 or int64, nullable) or two
+// non-NULL int32 key columns packed into one word, with no WHERE or a WHERE folded into an
+// interval (on the value column or on one other integer column).  A wavefront takes chunks of
+// kSChunk rows in parts of kSPart rows; lane l holds rows 2l and 2l+1 of every part, so every
+// load instruction reads one contiguous 512 B (4-byte column) or 1 KB (8-byte column) range
+// with 8- or 16-byte lanes, and a null bitmap costs one 32-bit load per part.  The loads are
+// buffer loads bounded by the resource's size: rows past the end read as zero and an absent
+// bitmap is a resource of size zero, so no load needs a bounds
+// test or a clamp, and the next chunk's loads are issued before the current chunk is
+// aggregated.  Offsets are 32-bit bytes: every column of the block stays below 2^31 bytes.
+
+constexpr int kSRows = 4;                 // rows per lane per chunk
+constexpr int kSParts = kSRows / 2;        // parts of a chunk: lane l holds rows 2l, 2l+1 of each
+constexpr int kSChunk = 64 * kSRows;      // rows per wavefront per chunk
+constexpr int kSPart = 128;               // rows per part
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int64_t bytes) {
+    // word 3 = 0x00020000: the untyped 32-bit data format of raw buffer accesses on gfx9
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// The 32-bit words of one lane's kSRows rows of a column of E-byte elements: row j of the lane
+// in w[j] (E = 4) or w[2j], w[2j+1] (E = 8).
+template <int E>
+struct LaneWords {
+    uint32_t w[E * kSRows / 4];
+};
+
+// Every part of a lane's rows, the first at byte offset off, the next kSPart * E bytes further
+// each (non-temporal: the columns are streamed once, and the lines of the cold regions being
+// filled should stay in the L2).
+__device__ __forceinline__ void ld_lane(__amdgpu_buffer_rsrc_t r, uint32_t off, LaneWords<4>& t) {
+#pragma unroll
+    for (int q = 0; q < kSParts; q++) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + q * kSPart * 4), 0, 2);
+        t.w[2 * q] = v[0];
+        t.w[2 * q + 1] = v[1];
     }
-    for (int q = tid; q < a.P; q += bd) a.r_count[region0 + q] = min(cursor[q], (uint32_t)a.cap);
-    if (tid == 0) g_count_new(*gp, nnew[0]);
+}
+__device__ __forceinline__ void ld_lane(__amdgpu_buffer_rsrc_t r, uint32_t off, LaneWords<8>& t) {
+#pragma unroll
+    for (int q = 0; q < kSParts; q++) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + q * kSPart * 8), 0, 2);
+        t.w[4 * q] = v[0];
+        t.w[4 * q + 1] = v[1];
+        t.w[4 * q + 2] = v[2];
+        t.w[4 * q + 3] = v[3];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T word_elem(const uint32_t* w, int j) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = (uint64_t)w[2 * j] | ((uint64_t)w[2 * j + 1] << 32);
+        return __builtin_bit_cast(T, u);
+    } else {
+        return __builtin_bit_cast(T, w[j]);
+    }
+}
+
+// A null bitmap's words for every part of a lane's rows: the lane's two rows of a part lie in
+// one 32-bit word (2l is even), each part's word kSPart / 32 words after the previous one's.
+struct LaneBits {
+    uint32_t w[kSParts];
+    __device__ __forceinline__ bool bit(int j, uint32_t sh) const {
+        return (w[j >> 1] >> (sh + (uint32_t)(j & 1))) & 1u;
+    }
+};
+__device__ __forceinline__ LaneBits no_bits() {
+    LaneBits b;
+#pragma unroll
+    for (int q = 0; q < kSParts; q++) b.w[q] = 0u;
+    return b;
+}
+__device__ __forceinline__ LaneBits ld_bits(__amdgpu_buffer_rsrc_t r, uint32_t boff) {
+    LaneBits b;
+#pragma unroll
+    for (int q = 0; q < kSParts; q++)
+        b.w[q] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(boff + q * (kSPart / 8)), 0, 0);
+    return b;
+}
+
+template <typename VT, int MODE, typename KT, int FP>
+__global__ void __launch_bounds__(kBlock) k_tp_phase1s(const GView* gp, KeyCols kc, const VT* __restrict__ vals,
+                                                       const uint64_t* __restrict__ vnull, PredSet preds,
+                                                       int32_t n, uint32_t fields, P1Args a) {
+    extern __shared__ __align__(16) uint64_t lds[];
+    constexpr bool HAS_V = VTraits<VT>::has;
+    constexpr bool PAIR = kPairKey<KT>;
+    using VV = typename std::conditional<HAS_V, VT, int>::type;
+    constexpr int KE = PAIR ? 4 : (int)sizeof(KT);          // bytes per row of a key column
+    constexpr int VE = HAS_V ? (int)sizeof(VV) : 4;         // bytes per row of the value column
+    const bool f_sum = FP == kFieldsSum ? true : (fields & F_SUM) != 0;
+    const bool f_min = FP == kFieldsSum ? false : (fields & F_MIN) != 0;
+    const bool f_max = FP == kFieldsSum ? false : (fields & F_MAX) != 0;
+    const int H = a.H, tid = threadIdx.x;
+    const int lane = tid & 63;
+    const uint32_t nb = (uint32_t)(H / kBucket);
+    const P1Lds<VV> L = p1_begin<VT, VV>(lds, a, f_sum, f_min, f_max);
+    const int32_t K = __builtin_amdgcn_readfirstlane(min(a.meta[0], a.kmax));
+
+    const int64_t bitmap_bytes = ((int64_t)n + 63) / 64 * 8;
+    const __amdgpu_buffer_rsrc_t rk0 = make_rsrc(kc.col[0], (int64_t)n * KE);
+    const __amdgpu_buffer_rsrc_t rk1 = make_rsrc(PAIR ? kc.col[1] : kc.col[0], PAIR ? (int64_t)n * KE : 0);
+    const __amdgpu_buffer_rsrc_t rkn = make_rsrc(kc.nm[0], (!PAIR && kc.nm[0]) ? bitmap_bytes : 0);
+    const __amdgpu_buffer_rsrc_t rv = make_rsrc(vals, HAS_V ? (int64_t)n * VE : 0);
+    const __amdgpu_buffer_rsrc_t rvn = make_rsrc(vnull, vnull ? bitmap_bytes : 0);
+    const int CE = preds.rwide ? 8 : 4;
+    const __amdgpu_buffer_rsrc_t rc = make_rsrc(preds.rcol, MODE == 4 ? (int64_t)n * CE : 0);
+    const __amdgpu_buffer_rsrc_t rcn = make_rsrc(preds.rnull, (MODE == 4 && preds.rnull) ? bitmap_bytes : 0);
+
+    struct Chunk {
+        LaneWords<KE> k0, k1;
+        LaneWords<VE> v;
+        LaneWords<8> c;
+        LaneBits vb, kb, cb;
+    };
+    // the chunk's loads; a chunk past the end reads zeros
+    auto load = [&](uint32_t chunk, Chunk& t) {
+        const uint32_t row0 = chunk * (uint32_t)kSChunk + (uint32_t)lane * 2;
+        ld_lane(rk0, row0 * KE, t.k0);
+        if constexpr (PAIR) ld_lane(rk1, row0 * KE, t.k1);
+        if constexpr (HAS_V) ld_lane(rv, row0 * VE, t.v);
+        const uint32_t boff = (row0 >> 5) * 4;
+        t.vb = ld_bits(rvn, boff);
+        t.kb = PAIR ? no_bits() : ld_bits(rkn, boff);
+        t.cb = no_bits();
+        if constexpr (MODE == 4) {
+            if (preds.rwide) {
+                ld_lane(rc, row0 * 8, t.c);
+            } else {
+                LaneWords<4> c4;
+                ld_lane(rc, row0 * 4, c4);
+#pragma unroll
+                for (int j = 0; j < kSRows; j++) t.c.w[j] = c4.w[j];
+            }
+            t.cb = ld_bits(rcn, boff);
+        }
+    };
+    P1Ring<VT, VV> ring;
+    auto process = [&](uint32_t chunk, const Chunk& t) {
+        const uint32_t row0 = chunk * (uint32_t)kSChunk + (uint32_t)lane * 2;
+        const uint32_t sh = row0 & 31;
+        uint64_t key[kSRows];
+        VV val[kSRows];
+        int ls[kSRows];
+        uint32_t pass = 0, vnl = 0, knl = 0;
+#pragma unroll
+        for (int j = 0; j < kSRows; j++) {
+            const bool vn = t.vb.bit(j, sh);
+            const bool kn = t.kb.bit(j, sh);
+            uint64_t kk;
+            if constexpr (PAIR)
+                kk = (uint64_t)t.k0.w[j] | ((uint64_t)t.k1.w[j] << 32);
+            else
+                kk = (uint64_t)(int64_t)word_elem<KT>(t.k0.w, j);
+            kk = kn ? 0ull : kk;
+            VV v = 0;
+            if constexpr (HAS_V) v = word_elem<VV>(t.v.w, j);
+            int64_t cv = 0;
+            if constexpr (MODE == 4)
+                cv = preds.rwide ? word_elem<int64_t>(t.c.w, j) : (int64_t)word_elem<int32_t>(t.c.w, j);
+            const bool cn = t.cb.bit(j, sh);
+            const bool in_block = row0 + (uint32_t)((j >> 1) * kSPart + (j & 1)) < (uint32_t)n;
+            const bool ok = in_block & row_passes<VT, MODE>(preds, 0u, nullptr, 0, v, vn, cv, cn);
+            key[j] = kk;
+            val[j] = v;
+            pass |= (uint32_t)ok << j;
+            vnl |= (uint32_t)vn << j;
+            knl |= (uint32_t)kn << j;
+        }
+        // every row's bucket is read first (a NULL key reads key 0's; the NULL key and the key
+        // equal to the empty marker then take their side slots by selection); with no heavy key
+        // the table is not read
+#pragma unroll
+        for (int j = 0; j < kSRows; j++) {
+            int l = K ? heavy_find(L.lkey, nb, key[j]) : -1;
+            l = key[j] == kEmpty ? H + 1 : l;
+            ls[j] = ((knl >> j) & 1) ? H : l;
+        }
+#pragma unroll
+        for (int j = 0; j < kSRows; j++)
+            if (((pass >> j) & 1) && ls[j] >= 0) p1_heavy_add<VT>(L, ls[j], val[j], (vnl >> j) & 1);
+        // cold rows with a value go to the ring; a NULL value takes the atomic path
+#pragma unroll
+        for (int j = 0; j < kSRows; j++) {
+            const bool vn = (vnl >> j) & 1;
+            const bool cold = ((pass >> j) & 1) & (ls[j] < 0);
+            if (cold && vn) p1_atomic_row<VT>(gp, L.nnew, fields, key[j], hash64(key[j]), val[j], true);
+            ring.push(L, a, gp, fields, cold && !vn, key[j], val[j]);
+        }
+    };
+    // chunks are dealt to the wavefronts of the grid in turn; each wavefront loads its next
+    // chunk before it aggregates the current one (two register sets, used alternately)
+    const uint32_t nchunks = (uint32_t)(((int64_t)n + kSChunk - 1) / kSChunk);
+    const uint32_t stride = gridDim.x * (uint32_t)(kBlock / 64);
+    uint32_t c = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kBlock / 64) + (tid >> 6)));
+    Chunk A, B;
+    load(c, A);
+    while (c < nchunks) {
+        load(c + stride, B);
+        process(c, A);
+        c += stride;
+        if (c >= nchunks) break;
+        load(c + stride, A);
+        process(c, B);
+        c += stride;
+    }
+    if (ring.cnt) ring.drain(L, a, gp, fields, ring.cnt);
+    p1_end<VTraits<VT>::real>(L, a, gp);
 }
 
 // ---- phase 2 -------------------------------------------------------------------------------
@@ -1351,11 +1623,7 @@ int launch_sample_m(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols&
     }
 }
 
-template <typename VT, int MODE, typename KT, int FP>
-int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc, const void* vals,
-              const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
-              int64_t n, uint32_t fields, hipStream_t s) {
-    // gfx950 launches up to 160 KB of LDS per workgroup without a function attribute
+P1Args p1_args(const TwoPhaseBufs& b) {
     P1Args a;
     a.image = b.p1_image;
     a.slot = b.p1_slot;
@@ -1374,50 +1642,72 @@ int launch_p1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols&
     a.zero = b.zero_word;
     a.part_shift = rec_shift_of(b.P);
     a.accumulate = b.pending_rows > 0 ? 1 : 0;
+    return a;
+}
+
+// gfx950 launches up to 160 KB of LDS per workgroup without a function attribute
+template <typename VT, int MODE>
+int launch_p1(TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc, const void* vals, const uint64_t* vnull,
+              const int8_t* mask, const PredSet& ps, uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
     const size_t lds = p1_lds_bytes(b.H1, b.P, b.vbytes, fields);
-    k_tp_phase1<VT, MODE, KT, FP><<<b.W, b.block, lds, s>>>(b.gview, ks, kc, (const VT*)vals, vnull,
-                                                            mask, ps, vpred, n, fields, a);
+    k_tp_phase1<VT, MODE, void, kFieldsAny><<<b.W, b.block, lds, s>>>(b.gview, ks, kc, (const VT*)vals, vnull,
+                                                                      mask, ps, vpred, n, fields, p1_args(b));
+    QK_HIP(hipGetLastError());
+    return QK_SUCCESS;
+}
+
+template <typename VT, int MODE, typename KT, int FP>
+int launch_p1s(TwoPhaseBufs& b, const KeyCols& kc, const void* vals, const uint64_t* vnull, const PredSet& ps,
+               int64_t n, uint32_t fields, hipStream_t s) {
+    const size_t lds = p1_lds_bytes(b.H1, b.P, b.vbytes, fields);
+    k_tp_phase1s<VT, MODE, KT, FP><<<b.W, b.block, lds, s>>>(b.gview, kc, (const VT*)vals, vnull, ps, (int32_t)n,
+                                                             fields, p1_args(b));
     QK_HIP(hipGetLastError());
     return QK_SUCCESS;
 }
 
 template <typename VT, typename KT, int FP>
-int launch_p1_m(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
-                const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
-                uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
-    switch (mode) {
-    case 0: return launch_p1<VT, 0, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    case 1: return launch_p1<VT, 1, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    case 2: return launch_p1<VT, 2, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    case 4: return launch_p1<VT, 4, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    default:
-        if constexpr (VTraits<VT>::has)
-            return launch_p1<VT, 3, KT, FP>(b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-        return fail(QK_ERROR, "value range filter without a value column");
-    }
+int launch_p1s_m(int mode, TwoPhaseBufs& b, const KeyCols& kc, const void* vals, const uint64_t* vnull,
+                 const PredSet& ps, int64_t n, uint32_t fields, hipStream_t s) {
+    if (mode == 0) return launch_p1s<VT, 0, KT, FP>(b, kc, vals, vnull, ps, n, fields, s);
+    if (mode == 4) return launch_p1s<VT, 4, KT, FP>(b, kc, vals, vnull, ps, n, fields, s);
+    if constexpr (VTraits<VT>::has) return launch_p1s<VT, 3, KT, FP>(b, kc, vals, vnull, ps, n, fields, s);
+    return fail(QK_ERROR, "value range filter without a value column");
 }
 
-// Typed fast paths for one int64 or int32 key column; any other key layout reads its key
-// through the generic key spec.
+// Blocks of at most this many rows take the streaming phase 1 (its loads use 32-bit byte
+// offsets: 8-byte columns and the prefetch past the end stay below 2^31 bytes).
+constexpr int64_t kStreamMaxRows = (int64_t)1 << 27;
+
+// The streaming form for one int64 or int32 key column, or two non-NULL int32 key columns
+// packed into one word, under no WHERE or a folded one; the general form reads any key layout
+// through the key spec and evaluates any WHERE.
 template <typename VT>
-int launch_p1_k(int mode, TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const KeyCols& kc,
-                const void* vals, const uint64_t* vnull, const int8_t* mask, const PredSet& ps,
-                uint32_t vpred, int64_t n, uint32_t fields, hipStream_t s) {
+int launch_p1_k(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc, const void* vals,
+                const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred, int64_t n,
+                uint32_t fields, hipStream_t s) {
     const bool sum_only = (fields & F_SUM) && !(fields & (F_MIN | F_MAX));
-    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_LONG) {
-        if (sum_only)
-            return launch_p1_m<VT, int64_t, kFieldsSum>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-        return launch_p1_m<VT, int64_t, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    const bool streamed = (mode == 0 || mode == 3 || mode == 4) && n <= kStreamMaxRows;
+    if (streamed && ks.single && ks.nkeys == 1 && ks.type[0] == QK_LONG) {
+        if (sum_only) return launch_p1s_m<VT, int64_t, kFieldsSum>(mode, b, kc, vals, vnull, ps, n, fields, s);
+        return launch_p1s_m<VT, int64_t, kFieldsAny>(mode, b, kc, vals, vnull, ps, n, fields, s);
     }
-    if (ks.single && ks.nkeys == 1 && ks.type[0] == QK_INT) {
-        if (sum_only)
-            return launch_p1_m<VT, int32_t, kFieldsSum>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-        return launch_p1_m<VT, int32_t, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    if (streamed && ks.single && ks.nkeys == 1 && ks.type[0] == QK_INT) {
+        if (sum_only) return launch_p1s_m<VT, int32_t, kFieldsSum>(mode, b, kc, vals, vnull, ps, n, fields, s);
+        return launch_p1s_m<VT, int32_t, kFieldsAny>(mode, b, kc, vals, vnull, ps, n, fields, s);
     }
-    if (!ks.single && ks.nkeys == 2 && ks.kw == 1 && ks.type[0] == QK_INT && ks.type[1] == QK_INT &&
+    if (streamed && !ks.single && ks.nkeys == 2 && ks.kw == 1 && ks.type[0] == QK_INT && ks.type[1] == QK_INT &&
         ks.word[0] == 0 && ks.shift[0] == 0 && ks.word[1] == 0 && ks.shift[1] == 32 && !kc.nm[0] && !kc.nm[1])
-        return launch_p1_m<VT, KeyPairI32, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
-    return launch_p1_m<VT, void, kFieldsAny>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        return launch_p1s_m<VT, KeyPairI32, kFieldsAny>(mode, b, kc, vals, vnull, ps, n, fields, s);
+    switch (mode) {
+    case 0: return launch_p1<VT, 0>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 1: return launch_p1<VT, 1>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 2: return launch_p1<VT, 2>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    case 4: return launch_p1<VT, 4>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+    default:
+        if constexpr (VTraits<VT>::has) return launch_p1<VT, 3>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        return fail(QK_ERROR, "value range filter without a value column");
+    }
 }
 
 template <typename VT>
@@ -1629,13 +1919,13 @@ int twophase_phase1(TwoPhaseBufs& b, const GView& g, const KeySpec& ks, const Ke
     if (ev0) QK_HIP(hipEventRecord(ev0, s));
     int st;
     if (!vals) {
-        st = launch_p1_k<NoValue>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
+        st = launch_p1_k<NoValue>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     } else {
         switch (vtype) {
-        case QK_INT: st = launch_p1_k<int32_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
-        case QK_LONG: st = launch_p1_k<int64_t>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
-        case QK_FLOAT: st = launch_p1_k<float>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
-        default: st = launch_p1_k<double>(mode, b, g, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_INT: st = launch_p1_k<int32_t>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_LONG: st = launch_p1_k<int64_t>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        case QK_FLOAT: st = launch_p1_k<float>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
+        default: st = launch_p1_k<double>(mode, b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s); break;
         }
     }
     if (st != QK_SUCCESS) return st;
