@@ -1354,9 +1354,11 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
 
     uint32_t nnew = 0;  // keys this thread added to the table
     // slot of a key (claimed when absent), probing from h & sub_mask as g_slot1 does
-    auto claim_from = [&](uint64_t key, uint32_t s) -> int {
+    // (free: slot s was seen empty, so the claim starts with the CAS)
+    auto claim_from = [&](uint64_t key, uint32_t s, bool free = false) -> int {
         for (uint32_t it = 0; it < S; it++) {
-            const uint64_t cur = okey[s];
+            const uint64_t cur = free ? kEmpty : okey[s];
+            free = false;
             if (cur == key) return (int)s;
             if (cur == kEmpty) {
                 const unsigned long long old = atomicCAS((unsigned long long*)&okey[s], (unsigned long long)kEmpty,
@@ -1427,7 +1429,6 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
     // so the run's counts and offsets are scalar; the next run's counts are loaded while this
     // run is walked.
     constexpr int kRegionsPerWave = 4;
-    constexpr int kWindow = 4;
     const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = bd >> 6;
     const int32_t stride = a.P * a.cap;  // between the regions of one partition (host-checked)
     int32_t cn[kRegionsPerWave];
@@ -1465,12 +1466,13 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
                 const int64_t at = rbase + off;
                 get_rec<VT>(a.r_rec, at, key[i], val[i]);
             }
-            // the kWindow slots from every row's home at once; a key present in the table sits
-            // there unless its chain is longer (no deletions, so any match is the key's)
-            // home slot, with bit 31 set for a row of another sub-table of the partition (the
-            // window below masks the bit away)
-            uint32_t home[kP2Rows];
+            // a window of four slots around every row's home, read as two 128-bit words from
+            // the even slot at or below it: a key present in the table sits there unless its
+            // chain is longer (no deletions, so any match is the key's slot).  Home slot, with
+            // bit 31 set for a row of another sub-table of the partition.
+            uint32_t home[kP2Rows], from[kP2Rows];
             int hit[kP2Rows];
+            uint32_t free_at = 0;  // bit i: from[i] was seen empty
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
                 const uint64_t h = hash64(key[i]);
@@ -1478,13 +1480,25 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
             }
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
-                int h = -1;
-#pragma unroll
-                for (int w = kWindow - 1; w >= 0; w--) {
-                    const uint32_t sw = (home[i] + (uint32_t)w) & (S - 1);
-                    h = okey[sw] == key[i] ? (int)sw : h;
-                }
+                const uint32_t b0 = home[i] & (S - 2);
+                const uint32_t b2 = (b0 + 2) & (S - 1), b3 = (b0 + 3) & (S - 1);
+                const ulonglong2 w0 = *reinterpret_cast<const ulonglong2*>(okey + b0);
+                const ulonglong2 w1 = *reinterpret_cast<const ulonglong2*>(okey + b2);
+                int h = w1.y == key[i] ? (int)b3 : -1;
+                h = w1.x == key[i] ? (int)b2 : h;
+                h = w0.y == key[i] ? (int)(b0 + 1) : h;
+                h = w0.x == key[i] ? (int)b0 : h;
                 hit[i] = h;
+                // no match: a claim starts at the window's first empty slot from home on (slot
+                // b0 lies before an odd home), or just past the window when it has none
+                uint32_t f = (b0 + 4) & (S - 1);
+                bool fr = false;
+                if (w1.y == kEmpty) { f = b3; fr = true; }
+                if (w1.x == kEmpty) { f = b2; fr = true; }
+                if (w0.y == kEmpty) { f = b0 + 1; fr = true; }
+                if (!(home[i] & 1) && w0.x == kEmpty) { f = b0; fr = true; }
+                from[i] = f;
+                free_at |= (uint32_t)fr << i;
             }
 #pragma unroll
             for (int i = 0; i < kP2Rows; i++) {
@@ -1492,7 +1506,7 @@ __global__ void __launch_bounds__(kBlock) k_tp_phase2_owned(GView g, uint32_t fi
                 Part pr;
                 if constexpr (HAS_V) pr = row_part<VT>(val[i], false); else pr = row_part<NoValue>(NoValue{}, false);
                 int s = hit[i];
-                if (s < 0) s = claim_from(key[i], home[i]);
+                if (s < 0) s = claim_from(key[i], from[i], (free_at >> i) & 1);
                 if (s >= 0) add(s, 1, 1, pr.sum, pr.mn, pr.mx);
                 else g_set_lost(g);
             }
