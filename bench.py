@@ -85,7 +85,7 @@ class Zipf:
     description = ("zipf-256M: int64 key Zipf(s=1.1) over 1e6 distinct, f32 lognormal(0,1) "
                    "value with 5% NULLs, WHERE value BETWEEN 0.5 AND 4.0, SUM/AVG/COUNT")
     dtype = "f32"
-    counters = "k_tp_phase1_zipf256M_counters.txt"
+    counters = "k_tp_phase1s_zipf256M_counters.txt"
     rel_aggs = (0, 1)  # SUM and AVG of float32 within FP_REL of the FP64 oracle
 
     def __init__(self, qk, n):
@@ -145,7 +145,7 @@ class Composite:
     description = ("composite-512M: keys (int32 a, int32 b) uniform [0,4096)^2, WHERE int64 ts in "
                    "a 30-day window of one year, SUM/MIN/MAX/COUNT of int64 value")
     dtype = "int64"
-    counters = "k_tp_phase1_composite512M_counters.txt"
+    counters = "k_tp_phase1s_composite512M_counters.txt"
     rel_aggs = ()
 
     def __init__(self, qk, n):
@@ -285,7 +285,7 @@ def run_query(W, qk, args, blk: int, steps: int, warmup: int, keep: bool = False
     kern_ms, launches = gb.profile()
     gb.set_profiling(False)
     stats = gb.stats()
-    kernel = "k_tp_phase1" if stats["path"] == 1 else "k_gb_rows"
+    kernel = ("k_tp_phase1s" if stats.get("p1_form") == 1 else "k_tp_phase1") if stats["path"] == 1 else "k_gb_rows"
     ms_per_step = elapsed * 1e3 / steps
     rows_per_s = n * steps / elapsed
     avg_launch_ms = kern_ms / max(1, launches)

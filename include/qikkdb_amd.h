@@ -409,7 +409,8 @@ int qk_gb_set_defer_rows(qk_groupby h, int64_t rows);
  * count threshold, out[3] rows written to partition regions by the last block, out[4]
  * phase-1 workgroups, out[5] partitions, out[6] rows per region, out[7] phase-1 LDS slots,
  * out[8] phase-2 LDS slots, out[9] heavy key limit, out[10] sub-tables of the table, out[11]
- * growths of a QK_GB_AUTO_GROW table.  n = entries to fill (at most 12). */
+ * growths of a QK_GB_AUTO_GROW table, out[12] phase-1 kernel of the last two-phase block (1
+ * streaming form, 0 general form).  n = entries to fill (at most 13). */
 int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream);
 
 /* ======================================================================================

@@ -1499,8 +1499,8 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
     if (!h || !out || n < 0) return fail(QK_ERROR, "qk_gb_stats: bad argument");
     hipStream_t s = as_stream(stream);
     QK_HIP(hipStreamSynchronize(s));
-    int64_t v[12] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax, h->parts,
-                     h->grows};
+    int64_t v[13] = {h->last_path, 0, 0, 0, h->tp.W, h->tp.P, h->tp.cap, h->tp.H1, h->tp.H2, h->tp.kmax, h->parts,
+                     h->grows, h->tp.last_form};
     if (h->tp.heavy_meta) {
         int32_t meta[2];
         QK_HIP(hipMemcpy(meta, h->tp.heavy_meta, 8, hipMemcpyDeviceToHost));
@@ -1512,7 +1512,7 @@ int qk_gb_stats(qk_groupby h, int64_t* out, int32_t n, void* stream) {
         QK_HIP(hipMemcpy(rc.data(), h->tp.r_count, rc.size() * 4, hipMemcpyDeviceToHost));
         for (uint32_t c : rc) v[3] += c;
     }
-    for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
+    for (int i = 0; i < n && i < 13; i++) out[i] = v[i];
     return QK_SUCCESS;
 }
 

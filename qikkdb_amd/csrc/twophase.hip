@@ -217,6 +217,9 @@ __global__ void __launch_bounds__(kHistBlock) k_tp_hist(const uint64_t* __restri
 
 // Smallest count t >= 2 such that at most kmax sampled keys were seen t times or more
 // (kHistBins threads: thread i holds bin kHistBins-1-i, so a prefix sum is a suffix sum).
+// When the keys seen t times or more hold less than 1/16 of the sampled rows, there is no
+// heavy set (t = INT32_MAX): a heavy lookup costs every row a little, and so few heavy rows do
+// not repay it.
 __global__ void __launch_bounds__(kHistBins) k_tp_threshold(const uint32_t* hist, int32_t kmax,
                                                             int32_t* meta) {
     __shared__ int32_t wsum[16];
@@ -228,8 +231,16 @@ __global__ void __launch_bounds__(kHistBins) k_tp_threshold(const uint32_t* hist
     const int32_t suffix = block_excl_scan(v, wsum, &total) + v;  // keys seen >= b times
     if (b >= 2 && suffix <= kmax) atomicMin(&best, b);
     __syncthreads();
+    const int32_t t = best;
+    // sampled rows in all and of the keys seen >= t times (the last bin counts its keys
+    // kHistBins - 1 times each, at most 2^18 rows in all)
+    const int64_t rows_b = (int64_t)b * v;
+    const int32_t rb = (int32_t)min<int64_t>(rows_b, 0x3fffffff);
+    int32_t all_rows, heavy_rows;
+    (void)block_excl_scan(rb, wsum, &all_rows);
+    (void)block_excl_scan(b >= t ? rb : 0, wsum, &heavy_rows);
     if (i == 0) {
-        meta[1] = best;
+        meta[1] = (int64_t)heavy_rows * 16 < (int64_t)all_rows ? 0x7fffffff : t;
         meta[0] = 0;
     }
 }
@@ -1702,6 +1713,7 @@ int launch_p1_k(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc,
                 uint32_t fields, hipStream_t s) {
     const bool sum_only = (fields & F_SUM) && !(fields & (F_MIN | F_MAX));
     const bool streamed = (mode == 0 || mode == 3 || mode == 4) && n <= kStreamMaxRows;
+    b.last_form = 1;
     if (streamed && ks.single && ks.nkeys == 1 && ks.type[0] == QK_LONG) {
         if (sum_only) return launch_p1s_m<VT, int64_t, kFieldsSum>(mode, b, kc, vals, vnull, ps, n, fields, s);
         return launch_p1s_m<VT, int64_t, kFieldsAny>(mode, b, kc, vals, vnull, ps, n, fields, s);
@@ -1713,6 +1725,7 @@ int launch_p1_k(int mode, TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc,
     if (streamed && !ks.single && ks.nkeys == 2 && ks.kw == 1 && ks.type[0] == QK_INT && ks.type[1] == QK_INT &&
         ks.word[0] == 0 && ks.shift[0] == 0 && ks.word[1] == 0 && ks.shift[1] == 32 && !kc.nm[0] && !kc.nm[1])
         return launch_p1s_m<VT, KeyPairI32, kFieldsAny>(mode, b, kc, vals, vnull, ps, n, fields, s);
+    b.last_form = 0;
     switch (mode) {
     case 0: return launch_p1<VT, 0>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);
     case 1: return launch_p1<VT, 1>(b, ks, kc, vals, vnull, mask, ps, vpred, n, fields, s);

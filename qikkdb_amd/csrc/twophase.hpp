@@ -58,6 +58,7 @@ struct TwoPhaseBufs {
     GView gview_host{};               // what gview holds
     bool gview_valid = false;
     bool heavy_ready = false;
+    int32_t last_form = 0;            // phase-1 kernel of the last block: 1 streaming, 0 general
 };
 
 constexpr int kHistBins = 1024;

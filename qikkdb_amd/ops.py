@@ -261,7 +261,7 @@ class GPUGroupBy:
     def stats(self, stream=None) -> dict:
         names = ["path", "heavy_keys", "heavy_threshold", "region_rows", "p1_workgroups",
                  "partitions", "region_cap", "p1_lds_slots", "p2_lds_slots", "heavy_limit", "sub_tables",
-                 "grows"]
+                 "grows", "p1_form"]
         v = (C.c_int64 * len(names))()
         call("qk_gb_stats", self.h, v, len(names), stream)
         return dict(zip(names, list(v)))

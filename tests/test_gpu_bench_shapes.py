@@ -49,7 +49,7 @@ def test_zipf_256M_all_groups_equal_oracle():
     w = bench.Zipf(qk, bench.Zipf.rows)
     try:
         got, stats = run_bench_query(w)
-        assert stats["path"] == 1 and stats["sub_tables"] == 256
+        assert stats["path"] == 1 and stats["sub_tables"] == 256 and stats["p1_form"] == 1
         want = w.cpu_query(O, w.n, threads())().results()
         assert got["count"] == want["count"] == 999_401
         compare_all_groups(got, want, rel_aggs=bench.Zipf.rel_aggs, rel=FP_REL)
@@ -69,7 +69,7 @@ def test_composite_512M_all_groups_equal_oracle(composite_table):
     print("case: composite-512M, 8 x 2^26-row blocks, one deferred phase 2 over 8192 sub-tables", flush=True)
     w = composite_table
     got, stats = run_bench_query(w)
-    assert stats["path"] == 1 and stats["sub_tables"] == 8192 and stats["partitions"] == 1024
+    assert stats["path"] == 1 and stats["sub_tables"] == 8192 and stats["partitions"] == 1024 and stats["p1_form"] == 1
     want = w.cpu_query(O, w.n, threads())().results()
     assert got["count"] == want["count"]
     assert 15_000_000 < got["count"] < 16_777_216
