@@ -641,3 +641,74 @@ def test_nan_and_signed_zero_keys_and_values(dt, path):
         assert np.allclose(x[fin], y[fin], rtol=FP_REL, atol=1e-3)
         assert np.array_equal(x[np.isinf(y)], y[np.isinf(y)])
     assert np.array_equal(a["values"][4], b["values"][4])
+
+
+# ---- the streaming phase 1 at the edges of its blocks ---------------------------------------
+
+STREAM_BLOCKS = [1, 63, 255, 256, 257, 4095, 4097, 65_537, 300_001]
+
+
+def run_blocks(aggs, keys, vals, kn, vn, where, extra, flags):
+    """One table fed blocks of the sizes in STREAM_BLOCKS (two-phase path forced), its groups
+    and stats; the same rows through the oracle in one go."""
+    n = keys[0].size
+    ktypes = [qk._abi.TYPE_OF[k.dtype] for k in keys]
+    gb = qk.GPUGroupBy(aggs, ktypes, qk._abi.TYPE_OF[vals.dtype], 200_000, 1, flags=flags)
+    gb.set_path(1, 0)
+    s = 0
+    for b in STREAM_BLOCKS:
+        e = min(n, s + b)
+        kcols = [col(k[s:e], None if kn is None or kn[i] is None else kn[i][s:e]) for i, k in enumerate(keys)]
+        vcol = col(vals[s:e], None if vn is None else vn[s:e])
+        cols = {"value": vcol}
+        for name, (arr, isnull) in extra.items():
+            cols[name] = col(arr[s:e], None if isnull is None else isnull[s:e])
+        knm = [c.nulls for c in kcols] if kn is not None else None
+        preds = [w.gpu(cols, {}) for w in where]
+        gb.ProcessBlockWhere([c.data for c in kcols], vcol.data, e - s, preds, knm, vcol.nulls)
+        s = e
+    assert s == n
+    got = gb.results_host()
+    st = gb.stats()
+    gb.close()
+    want = cpu_group_by(aggs, keys, vals, kn, vn, where=where, extra_cols=extra)
+    return got, want, st
+
+
+@pytest.mark.parametrize("flags", [0, qk.GB_ASYNC])
+@pytest.mark.parametrize("form", ["long_value_range", "int_no_where", "pair_column_range"])
+def test_streaming_phase1_ragged_blocks(form, flags):
+    """Blocks of 1 row up to 300,001 rows, none a multiple of a chunk except 256, through the
+    streaming phase 1 (rows past a block's end read as zeros and must drop out): nullable int64
+    keys with the empty-marker key and a WHERE folded on the value; nullable int32 keys without
+    a WHERE; two int32 keys declared non-NULL (packed into one word) with a WHERE folded on
+    another, nullable column."""
+    rng = np.random.default_rng({"long_value_range": 1, "int_no_where": 2, "pair_column_range": 3}[form])
+    n = sum(STREAM_BLOCKS)
+    vn = rng.random(n) < 0.05
+    extra, where, kn = {}, [], None
+    if form == "long_value_range":
+        keys = [np.where(rng.random(n) < 0.6, datagen.zipf_keys(3, 1.1, 3000, n),
+                         rng.integers(-(2**40), 2**40, n)).astype(np.int64)]
+        keys[0][rng.random(n) < 0.01] = np.iinfo(np.int64).min
+        kn = [rng.random(n) < 0.01]
+        vals = rng.lognormal(0, 1, n).astype(np.float32)
+        where = [Where(qk.GREATER_EQUAL, "value", 0.5, qk.FLOAT), Where(qk.LESS_EQUAL, "value", 4.0, qk.FLOAT)]
+        aggs, real = [qk.AGG_SUM, qk.AGG_AVG, qk.AGG_COUNT, qk.AGG_COUNT_ALL], (0, 1)
+    elif form == "int_no_where":
+        keys = [datagen.zipf_keys(4, 1.1, 50_000, n).astype(np.int32)]
+        kn = [rng.random(n) < 0.02]
+        vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+        aggs, real = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT, qk.AGG_COUNT_ALL], ()
+    else:
+        keys = [rng.integers(0, 300, n).astype(np.int32), rng.integers(-(2**31), 2**31, n).astype(np.int32)]
+        keys[1][rng.random(n) < 0.5] = 7
+        vals = rng.integers(-10**9, 10**9, n).astype(np.int64)
+        ts = rng.integers(0, 10**6, n).astype(np.int64)
+        extra = {"ts": (ts, rng.random(n) < 0.03)}
+        where = [Where(qk.GREATER_EQUAL, "ts", 100_000, qk.LONG), Where(qk.LESS, "ts", 700_000, qk.LONG)]
+        aggs, real = [qk.AGG_SUM, qk.AGG_MIN, qk.AGG_MAX, qk.AGG_COUNT], ()
+        flags |= qk.GB_KEYS_NOT_NULL  # the two int32 fields then pack into one key word
+    got, want, st = run_blocks(aggs, keys, vals, kn, vn, where, extra, flags)
+    assert st["path"] == 1 and st["p1_form"] == 1
+    compare(got, want, len(aggs), real)
