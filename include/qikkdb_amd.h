@@ -513,7 +513,9 @@ int qk_batch_key_column(const int32_t* a, const int32_t* b, int32_t b_range, int
 typedef struct qk_serve_config {
     int32_t engines;          /* batch engines: batches running or holding results (0 = 2) */
     int32_t max_batch;        /* most requests per batch (0 = 4096) */
-    int64_t memory_cap;       /* projected bytes admitted at once (0 = 200 GB) */
+    int64_t memory_cap;       /* projected bytes admitted at once (0 = 200 GB); the engines' row
+                                 pools share the device memory left over it (scratch_rows is
+                                 lowered to fit, QK_OUT_OF_MEMORY when under 2^20 rows each) */
     double wait_ms;           /* with a batch running, the next one starts when its oldest
                                  request has waited this long (0 = half the last batch's time) */
     qk_batch_tuning tuning;   /* of every engine (tile_rows 0 = defaults, no graph replay) */

@@ -353,6 +353,25 @@ int qk_serve_create(qk_serve* out, const qk_batch_columns* cols, const qk_serve_
     }
     if (c.engines < 1 || c.engines > 8 || c.max_batch < 1 || c.memory_cap < 1 || c.wait_ms < 0)
         return qk::fail(QK_ERROR, "qk_serve_create: engines 1..8, max_batch >= 1, memory_cap >= 1, wait_ms >= 0");
+    {
+        // Each engine keeps its row pool (12 bytes per pooled row, grown to the largest batch it
+        // ran) beside the bytes the cap admits, so the engines share what the device has free
+        // over the cap, less a tenth left to the runtime (at their default of 2^32 rows, four
+        // engines' pools would not fit beside a 200 GB cap on 288 GB).
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            (void)hipGetLastError();
+            return qk::fail(QK_RUNTIME_ERROR, "qk_serve_create: hipMemGetInfo failed");
+        }
+        const int64_t spare = (int64_t)free_b - c.memory_cap;
+        const int64_t rows = spare > 0 ? spare / 10 * 9 / c.engines / 12 : 0;
+        if (rows < ((int64_t)1 << 20))
+            return qk::fail(QK_OUT_OF_MEMORY,
+                            "qk_serve_create: a memory cap of %lld bytes leaves %lld of the %zu free bytes for "
+                            "the row pools of %d engines", (long long)c.memory_cap, (long long)std::max<int64_t>(spare, 0),
+                            free_b, c.engines);
+        if (c.tuning.scratch_rows <= 0 || c.tuning.scratch_rows > rows) c.tuning.scratch_rows = rows;
+    }
     qk_serve_s* s = new (std::nothrow) qk_serve_s();
     if (!s) return qk::fail(QK_OUT_OF_MEMORY, "qk_serve_create: host allocation");
     s->cols = *cols;
