@@ -251,6 +251,17 @@ __global__ void k_gb_init(GView g, uint32_t fields) {
     }
 }
 
+// Clears n words in each of up to kMaxKeys + kMaxAggs buffers (blockIdx.y picks the buffer).
+struct ZeroList {
+    int32_t n;
+    uint64_t* p[kMaxKeys + kMaxAggs];
+};
+__global__ void k_zero_words(ZeroList z, int64_t n) {
+    uint64_t* p = z.p[blockIdx.y];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 // ---- results ------------------------------------------------------------------------------
 struct OutSpec {
     int32_t naggs;
@@ -1247,9 +1258,33 @@ int get_results_impl(qk_groupby h, int32_t naggs, const int32_t* aggs, const int
         if (out_value_nulls) out_value_nulls[a] = nullptr;
     }
     QK_TRY(flush_pending(h, s));
-    QK_TRY(take_error(h, s));
-    int64_t groups = 0;
-    QK_TRY(qk_gb_group_count(h, &groups, stream));
+    // the latched error, the key count and (single-word keys) the NULL group's slot word in
+    // one round trip to the host
+    TableCounters c;
+    bool null_group = false;
+    {
+        void* hb = nullptr;
+        QK_TRY(pinned_host(64, &hb));
+        QK_HIP(hipMemcpyAsync(hb, h->ctr, sizeof(TableCounters), hipMemcpyDeviceToHost, s));
+        if (h->ks.kw == 1)
+            QK_HIP(hipMemcpyAsync((char*)hb + 32, h->keys + h->cap, 8, hipMemcpyDeviceToHost, s));
+        QK_HIP(hipStreamSynchronize(s));
+        static_assert(sizeof(TableCounters) <= 32, "counters and slot word share the staging buffer");
+        memcpy(&c, hb, sizeof(TableCounters));
+        if (h->ks.kw == 1) {
+            uint64_t w;
+            memcpy(&w, (char*)hb + 32, 8);
+            null_group = w != 0;
+        }
+    }
+    if (c.err != 0) {  // read and cleared, as take_error does
+        QK_HIP(hipMemsetAsync(&h->ctr->err, 0, sizeof(int32_t), s));
+        if (c.err == QK_HASH_TABLE_FULL)
+            return fail(QK_HASH_TABLE_FULL, "GROUP BY hash table is full (%llu keys, capacity %llu)",
+                        (unsigned long long)c.nkeys, (unsigned long long)h->logical_cap);
+        return fail(c.err, "GROUP BY kernel error %d", c.err);
+    }
+    const int64_t groups = (int64_t)c.nkeys + (null_group ? 1 : 0);
     if (groups == 0) return QK_SUCCESS;
     if (groups > INT32_MAX) return fail(QK_ERROR, "qk_gb_get_results: too many groups");
 
@@ -1269,24 +1304,31 @@ int get_results_impl(qk_groupby h, int32_t naggs, const int32_t* aggs, const int
     int st = QK_SUCCESS;
     for (int c = 0; c < h->nkeys && st == QK_SUCCESS; c++) {
         st = alloc((size_t)groups * type_size(h->key_types[c]), &os.kout[c]);
-        if (st == QK_SUCCESS && out_key_nulls) {
-            st = alloc(nwords * 8, (void**)&os.konull[c]);
-            if (st == QK_SUCCESS) st = hipMemsetAsync(os.konull[c], 0, nwords * 8, s) == hipSuccess ? QK_SUCCESS : QK_RUNTIME_ERROR;
-        }
+        if (st == QK_SUCCESS && out_key_nulls) st = alloc(nwords * 8, (void**)&os.konull[c]);
     }
     for (int a = 0; a < naggs && st == QK_SUCCESS; a++) {
         os.agg[a] = aggs[a];
         os.otype[a] = otypes[a];
         if (aggs[a] == QK_AGG_NONE || !out_values) continue;
         st = alloc((size_t)groups * type_size(otypes[a]), &os.out[a]);
-        if (st == QK_SUCCESS && out_value_nulls) {
-            st = alloc(nwords * 8, (void**)&os.onull[a]);
-            if (st == QK_SUCCESS) st = hipMemsetAsync(os.onull[a], 0, nwords * 8, s) == hipSuccess ? QK_SUCCESS : QK_RUNTIME_ERROR;
+        if (st == QK_SUCCESS && out_value_nulls) st = alloc(nwords * 8, (void**)&os.onull[a]);
+    }
+    if (st == QK_SUCCESS) {
+        // every null mask cleared by one launch (emit only sets bits)
+        ZeroList zl{};
+        for (int c = 0; c < h->nkeys; c++)
+            if (os.konull[c]) zl.p[zl.n++] = os.konull[c];
+        for (int a = 0; a < naggs; a++)
+            if (os.onull[a]) zl.p[zl.n++] = os.onull[a];
+        if (zl.n) {
+            const int64_t grid = std::min<int64_t>(((int64_t)nwords + 255) / 256, (int64_t)device_info().cus * 4);
+            k_zero_words<<<dim3((unsigned)grid, (unsigned)zl.n), 256, 0, s>>>(zl, (int64_t)nwords);
+            if (hipGetLastError() != hipSuccess) st = QK_RUNTIME_ERROR;
         }
     }
     if (st != QK_SUCCESS) {
         cleanup();
-        return st == QK_RUNTIME_ERROR ? fail(st, "qk_gb_get_results: memset failed") : st;
+        return st == QK_RUNTIME_ERROR ? fail(st, "qk_gb_get_results: clearing the null masks failed") : st;
     }
     const int64_t nslots = (int64_t)h->cap + 2;
     const int64_t tiles = (nslots + kEmitTile - 1) / kEmitTile;
