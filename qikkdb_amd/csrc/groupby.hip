@@ -239,6 +239,11 @@ __global__ void __launch_bounds__(1024) k_gb_rows(GView g, KeySpec ks, KeyCols k
 
 // ---- table initialisation ----------------------------------------------------------------
 __global__ void k_gb_init(GView g, uint32_t fields) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the table's counters (TableCounters)
+        *g.nkeys = 0;
+        *g.err = 0;
+        *g.lost = 0;
+    }
     const uint64_t total = g.cap + 2;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total; s += stride) {
@@ -758,7 +763,6 @@ int init_table(qk_groupby_s* h, hipStream_t s) {
     if (grid > capg) grid = capg;
     k_gb_init<<<(unsigned)grid, 256, 0, s>>>(g, h->fields);
     QK_HIP(hipGetLastError());
-    QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
     if (h->seed_slot || h->seed_bits) {  // a reset drops the seeded keys too
         QK_HIP(hipStreamSynchronize(s));
         if (h->seed_slot) pool_free(h->seed_slot);
@@ -843,7 +847,6 @@ int grow_table(qk_groupby_s* h, uint64_t need, hipStream_t s) {
     const unsigned igrid = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)device_info().cus * 16);
     k_gb_init<<<igrid, 256, 0, s>>>(g, h->fields);
     QK_HIP(hipGetLastError());
-    QK_HIP(hipMemsetAsync(h->ctr, 0, sizeof(TableCounters), s));
     const unsigned mgrid = (unsigned)std::min<uint64_t>((old.cap + 2 + 255) / 256, (uint64_t)device_info().cus * 16);
     switch (h->ks.kw) {
     case 1: k_gb_merge<1><<<mgrid, 256, 0, s>>>(g, old, h->fields); break;
