@@ -370,11 +370,19 @@ __global__ void __launch_bounds__(1024) k_tp_order(const uint64_t* __restrict__ 
             pcls[i] = 31 - __builtin_clz(max(ocnt[k], 1u));
         }
     }
-    for (int cls = 31; cls >= 0; cls--) {
-        bool mine = false;
+    // the classes present, so that only they take a barrier each
+    __shared__ uint32_t s_classes;
+    if (t == 0) s_classes = 0;
+    __syncthreads();
+    uint32_t my_classes = 0;
 #pragma unroll
-        for (int i = 0; i < kPlace; i++) mine = mine || pcls[i] == cls;
-        if (!__syncthreads_or(mine)) continue;
+    for (int i = 0; i < kPlace; i++) my_classes |= pcls[i] >= 0 ? 1u << pcls[i] : 0u;
+    if (my_classes) atomicOr(&s_classes, my_classes);
+    __syncthreads();
+    const uint32_t classes = s_classes;
+    for (int cls = 31; cls >= 0; cls--) {
+        if (!((classes >> cls) & 1u)) continue;
+        __syncthreads();  // the keys of the classes above are placed
 #pragma unroll
         for (int i = 0; i < kPlace; i++) {
             if (pcls[i] != cls) continue;
