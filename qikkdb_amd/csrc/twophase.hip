@@ -18,6 +18,9 @@ constexpr int kP2Rows = 6;    // cold rows per thread per pass in phase 2
 constexpr int kBlock = 1024;
 constexpr int kBucket = 4;    // slots per bucket of the phase-1 heavy table
 constexpr int kSampleLds = 2048;
+// threads per sampling workgroup: each flushes kSampleLds / kSampleBlock slots of its LDS
+// table into the device table, one probe at a time
+constexpr int kSampleBlock = 1024;
 
 // A cold row in its region: the key and the value packed into one record of 32-bit words
 // (8, 12 or 16 bytes), so a workgroup fills one line per partition at a time instead of one
@@ -129,7 +132,7 @@ __device__ __forceinline__ void sample_add(uint64_t* skeys, uint32_t* scnt, uint
 // Each workgroup counts a contiguous range of samples in LDS first, so that a hot key costs
 // one device atomic per workgroup instead of one per sample.
 template <typename VT, int MODE>
-__global__ void __launch_bounds__(256) k_tp_sample(KeySpec ks, KeyCols kc, const VT* __restrict__ vals,
+__global__ void __launch_bounds__(kSampleBlock) k_tp_sample(KeySpec ks, KeyCols kc, const VT* __restrict__ vals,
                                                    const uint64_t* __restrict__ vnull,
                                                    const int8_t* __restrict__ mask, PredSet preds,
                                                    uint32_t vpred, int64_t n, int64_t stride,
@@ -1633,7 +1636,7 @@ template <typename VT, int MODE>
 int launch_sample(TwoPhaseBufs& b, const KeySpec& ks, const KeyCols& kc, const void* vals,
                   const uint64_t* vnull, const int8_t* mask, const PredSet& ps, uint32_t vpred,
                   int64_t n, int64_t nsamples, int64_t stride, uint64_t smask, hipStream_t s) {
-    k_tp_sample<VT, MODE><<<grid_cap(nsamples, 1024, device_info().cus), 256, 0, s>>>(
+    k_tp_sample<VT, MODE><<<grid_cap(nsamples, 1024, device_info().cus), kSampleBlock, 0, s>>>(
         ks, kc, (const VT*)vals, vnull, mask, ps, vpred, n, stride, nsamples, b.sample_keys,
         b.sample_cnt, smask);
     QK_HIP(hipGetLastError());
