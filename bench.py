@@ -665,7 +665,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by row blocks across GPUs by design, but only the single-GPU "
               "kernel is built and measured)", file=sys.stderr)
         return 2

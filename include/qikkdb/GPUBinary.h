@@ -40,7 +40,7 @@ template <typename OP, typename OUT, typename L, typename R>
 class GPUBinary {
 public:
     static_assert(std::is_same<OUT, int8_t>::value,
-                  "this build provides the filter (int8 mask) form of GPUBinary");
+                  "qikkdb_amd provides the filter (int8 mask) form of GPUBinary");
     static void Binary(OUT* output, L ACol, R BCol, int32_t dataElementCount, nullmask_t* nullBitMask = nullptr) {
         using A = qikkdb::Operand<L>;
         using B = qikkdb::Operand<R>;

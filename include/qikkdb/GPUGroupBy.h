@@ -50,7 +50,7 @@ void merge_tables(T* self, qk_groupby self_h, std::vector<std::unique_ptr<IGroup
 }
 
 // The typing rules of GPUGroupBy.cuh:522-529 and :563-567: COUNT gives int64, every other
-// aggregate gives the value type (this build also accepts a widened SUM: int32 -> int64,
+// aggregate gives the value type (the library also accepts a widened SUM: int32 -> int64,
 // float -> double).
 template <typename AGG, typename O, typename V>
 constexpr void check_output_type() {

@@ -119,7 +119,7 @@ public:
 
     template <typename OUT, typename IN>
     static void Sum(OUT& outValueHost, IN* inputBuffer, const int32_t dataElementCount) {
-        static_assert(sizeof(IN) == 1, "this build sums byte masks");
+        static_assert(sizeof(IN) == 1, "SumKeep sums byte masks");
         int64_t s = 0;
         qikkdb::check(qk_mask_sum(reinterpret_cast<const int8_t*>(inputBuffer), dataElementCount, &s, nullptr));
         outValueHost = static_cast<OUT>(s);
